@@ -1,0 +1,49 @@
+"""Builds the native parts of fft3dcl_amd in-tree.
+
+    libfft3d_hip.so   the gfx950 kernels + C ABI (include/fft3d_hip.h)
+    libfft3dcl_vs.so  the VapourSynth API 4 plugin over that C ABI
+
+Only hipcc / g++ are used; nothing is fetched. Run `python -m fft3dcl_amd.build`.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+HIP_LIB = PKG / "libfft3d_hip.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# Packed FP32 instructions are disabled for every HIP source of this package.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+             "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd[:3]) + " ...")
+
+
+def build_hip(out: Path = HIP_LIB) -> Path:
+    src = CSRC / "fft3d_capi.hip"
+    tmp = out.with_suffix(".so.tmp")
+    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", str(src), "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> list[Path]:
+    return [build_hip()]
+
+
+if __name__ == "__main__":
+    for p in build_all():
+        print(p)

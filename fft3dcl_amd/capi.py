@@ -1,0 +1,226 @@
+"""ctypes binding of the C ABI in include/fft3d_hip.h (libfft3d_hip.so).
+
+Mirrors the reference's GPU backend object (GpuRef, src/cl/gpu.h:16-88) as the
+class `Plane`; the clip-level scheduling of its VapourSynth filter
+(src/fft3dcl.cpp:311-526) is `fft3dcl_amd.filter.FFT3DCL`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+DEFAULT_LIB = PKG / "libfft3d_hip.so"
+
+FFT3D_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "ELAUNCH", -5: "EMISSING",
+          -6: "EUNSUPPORTED"}
+
+PARAM_FIELDS = [
+    ("sigma", C.c_float), ("beta", C.c_float), ("bw", C.c_int), ("bh", C.c_int),
+    ("bt", C.c_int), ("sharpen", C.c_float), ("plane", C.c_int), ("mode", C.c_int),
+    ("bordersize", C.c_int), ("precision", C.c_int), ("degrid", C.c_float),
+    ("scutoff", C.c_float), ("svr", C.c_float), ("smin", C.c_float), ("smax", C.c_float),
+    ("kratio", C.c_float), ("ow", C.c_int), ("oh", C.c_int), ("wintype", C.c_int),
+    ("interlaced", C.c_int), ("sigma2", C.c_float), ("sigma3", C.c_float),
+    ("sigma4", C.c_float), ("oldfft", C.c_int),
+]
+PARAM_NAMES = [n for n, _ in PARAM_FIELDS]
+
+
+class Params(C.Structure):
+    _fields_ = PARAM_FIELDS
+
+
+class PlaneDesc(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("bits", C.c_int),
+                ("sample_type", C.c_int), ("chroma", C.c_int)]
+
+
+class PlaneInfo(C.Structure):
+    _fields_ = [("nb", C.c_int), ("bw", C.c_int), ("bh", C.c_int), ("ow", C.c_int),
+                ("oh", C.c_int), ("mode", C.c_int), ("fbw", C.c_int), ("bm", C.c_int),
+                ("bt", C.c_int), ("current", C.c_int), ("ncols", C.c_int),
+                ("nrows", C.c_int), ("blocks_per_wg", C.c_int),
+                ("spectrum_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
+                ("cached_frames", C.c_int)]
+
+
+EXPORTS = [
+    "fft3d_params_default", "fft3d_dev_init", "fft3d_plane_create", "fft3d_plane_destroy",
+    "fft3d_plane_error", "fft3d_plane_has", "fft3d_plane_forward", "fft3d_plane_process",
+    "fft3d_plane_prune", "fft3d_plane_clear", "fft3d_live_instances", "fft3d_live_bytes",
+    "fft3d_plane_get_info", "fft3d_plane_get_table", "fft3d_plane_read_spectrum",
+]
+
+
+class FFT3DError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+_LIBS: dict[str, C.CDLL] = {}
+
+
+def load(path: str | Path | None = None) -> C.CDLL:
+    """Loads (once) and types the C ABI library. Raises if it is missing: the
+    GPU path has no fallback."""
+    path = str(path or DEFAULT_LIB)
+    if path in _LIBS:
+        return _LIBS[path]
+    if not Path(path).exists():
+        raise FileNotFoundError(
+            f"{path} is not built; run `python -m fft3dcl_amd.build` (the GPU path has "
+            "no CPU fallback)")
+    L = C.CDLL(path)
+    vp, i, sz = C.c_void_p, C.c_int, C.c_size_t
+    L.fft3d_params_default.argtypes = [C.POINTER(Params)]
+    L.fft3d_params_default.restype = None
+    L.fft3d_dev_init.argtypes = [i, C.c_char_p, sz]
+    L.fft3d_plane_create.argtypes = [C.POINTER(Params), C.POINTER(PlaneDesc),
+                                     C.POINTER(vp), C.c_char_p, sz]
+    L.fft3d_plane_destroy.argtypes = [vp]
+    L.fft3d_plane_destroy.restype = None
+    L.fft3d_plane_error.argtypes = [vp]
+    L.fft3d_plane_error.restype = C.c_char_p
+    L.fft3d_plane_has.argtypes = [vp, i]
+    L.fft3d_plane_forward.argtypes = [vp, i, vp, sz, i, vp]
+    L.fft3d_plane_process.argtypes = [vp, C.POINTER(C.c_int), i, vp, sz, i, vp]
+    L.fft3d_plane_prune.argtypes = [vp, i, i]
+    L.fft3d_plane_prune.restype = None
+    L.fft3d_plane_clear.argtypes = [vp]
+    L.fft3d_plane_clear.restype = None
+    L.fft3d_live_instances.argtypes = []
+    L.fft3d_live_bytes.argtypes = []
+    L.fft3d_live_bytes.restype = sz
+    L.fft3d_plane_get_info.argtypes = [vp, C.POINTER(PlaneInfo)]
+    L.fft3d_plane_get_table.argtypes = [vp, i, vp]
+    L.fft3d_plane_read_spectrum.argtypes = [vp, i, vp]
+    _LIBS[path] = L
+    return L
+
+
+def make_params(lib: C.CDLL | None = None, **kw) -> Params:
+    """Script parameters with the reference's key-dependent defaults
+    (src/fft3dcl.cpp:171-195): degrid by mode, ow/oh by bw/bh, sigma2/3/4 by
+    sigma."""
+    p = Params()
+    bw = kw.get("bw", 32)
+    bh = kw.get("bh", 32)
+    sigma = kw.get("sigma", 2.0)
+    mode = kw.get("mode", 1)
+    d = dict(sigma=sigma, beta=1.0, bw=bw, bh=bh, bt=3, sharpen=0.0, plane=0, mode=mode,
+             bordersize=1, precision=0, degrid=1.0 if mode == 1 else 0.0, scutoff=0.3,
+             svr=1.0, smin=4.0, smax=20.0, kratio=2.0, ow=bw // 2, oh=bh // 2, wintype=0,
+             interlaced=0, sigma2=sigma, sigma3=sigma, sigma4=sigma, oldfft=-1)
+    for k, v in kw.items():
+        if k not in d:
+            raise TypeError(f"FFT3DCL: unknown parameter '{k}'")
+        d[k] = v
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(x) -> int:
+    """Address of a numpy array or a torch tensor (device or host)."""
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    return int(x.data_ptr())
+
+
+class Plane:
+    """One processed plane of one filter instance (the GpuRef counterpart)."""
+
+    def __init__(self, params: Params, width: int, height: int, bits: int,
+                 sample_type: int = 0, chroma: bool = False, lib=None):
+        self.lib = load(lib)
+        self.desc = PlaneDesc(width, height, bits, sample_type, int(bool(chroma)))
+        self.bps = 4 if sample_type == 1 else (1 if bits <= 8 else 2)
+        self.dtype = np.float32 if sample_type == 1 else (np.uint8 if bits <= 8 else np.uint16)
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = self.lib.fft3d_plane_create(C.byref(params), C.byref(self.desc), C.byref(h), err,
+                                         512)
+        if rc != FFT3D_OK:
+            raise FFT3DError(rc, err.value.decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fft3d_plane_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def error(self) -> str:
+        return self.lib.fft3d_plane_error(self.h).decode()
+
+    def _check(self, rc: int):
+        if rc != FFT3D_OK:
+            raise FFT3DError(rc, f"{ERRORS.get(rc, rc)}: {self.error}")
+
+    @property
+    def info(self) -> PlaneInfo:
+        i = PlaneInfo()
+        self._check(self.lib.fft3d_plane_get_info(self.h, C.byref(i)))
+        return i
+
+    @property
+    def nbytes(self) -> int:
+        return self.desc.width * self.desc.height * self.bps
+
+    def has(self, fn: int) -> bool:
+        return bool(self.lib.fft3d_plane_has(self.h, fn))
+
+    def forward(self, fn: int, src, on_device: bool = False, stream: int | None = None):
+        self._check(self.lib.fft3d_plane_forward(self.h, fn, _ptr(src), self.nbytes,
+                                                 int(on_device), stream))
+
+    def process(self, window, dst, on_device: bool = False, stream: int | None = None):
+        w = (C.c_int * len(window))(*window)
+        self._check(self.lib.fft3d_plane_process(self.h, w, len(window), _ptr(dst),
+                                                 self.nbytes, int(on_device), stream))
+
+    def prune(self, lo: int, hi: int):
+        self.lib.fft3d_plane_prune(self.h, lo, hi)
+
+    def clear(self):
+        self.lib.fft3d_plane_clear(self.h)
+
+    def table(self, which: int) -> np.ndarray | None:
+        n = self.lib.fft3d_plane_get_table(self.h, which, None)
+        if n <= 0:
+            return None
+        a = np.zeros(n, np.float32)
+        self.lib.fft3d_plane_get_table(self.h, which, a.ctypes.data)
+        return a
+
+    def read_spectrum(self, fn: int) -> np.ndarray:
+        i = self.info
+        a = np.zeros((i.nb, i.bh, i.fbw, 2), np.float32)
+        self._check(self.lib.fft3d_plane_read_spectrum(self.h, fn, a.ctypes.data))
+        return a[..., 0] + 1j * a[..., 1]
+
+
+def live_instances(lib=None) -> int:
+    return load(lib).fft3d_live_instances()
+
+
+def live_bytes(lib=None) -> int:
+    return load(lib).fft3d_live_bytes()
+
+
+def dev_init(device: int = 0, lib=None):
+    err = C.create_string_buffer(256)
+    rc = load(lib).fft3d_dev_init(device, err, 256)
+    if rc != FFT3D_OK:
+        raise FFT3DError(rc, err.value.decode())

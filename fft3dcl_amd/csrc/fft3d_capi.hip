@@ -1,0 +1,502 @@
+// C ABI of the MI355X backend (include/fft3d_hip.h): plane objects, the
+// device-resident spectra ring, and the launches of the gfx950 kernels.
+//
+// Replaces the reference's GpuRef host class (src/cl/gpu.cpp). Differences
+// by design: tables and grid are built here from the validated parameters
+// (no CPU backend object is needed); the Kalman first-frame init is folded
+// into the filter launch; spectra are stored packed (fk_kernels.h).
+#include "fk_kernels.h"
+#include "fft3d_params.h"
+#include "fft3d_hip.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+std::atomic<int> g_live{0};
+std::atomic<size_t> g_bytes{0};
+
+using AnalysisFn = void (*)(fk::PlaneArgs, const void*, fk::c32*);
+using FilterFn = void (*)(fk::PlaneArgs, fk::WindowArgs, float4*, int, float*);
+using SynthFn = void (*)(fk::PlaneArgs, const float*, void*);
+
+struct KernelSet {
+  AnalysisFn analysis = nullptr;
+  FilterFn filter_inverse = nullptr;
+  int G = 0;
+};
+
+template <int BW, int BH>
+KernelSet make_set() {
+  KernelSet k;
+  k.analysis = &fk::k_analysis<BW, BH>;
+  k.filter_inverse = &fk::k_filter_inverse<BW, BH>;
+  k.G = fk::Shape<BW, BH>::G;
+  return k;
+}
+
+template <int BW>
+KernelSet set_for_bh(int bh) {
+  switch (bh) {
+  case 4: return make_set<BW, 4>();
+  case 8: return make_set<BW, 8>();
+  case 16: return make_set<BW, 16>();
+  case 32: return make_set<BW, 32>();
+  case 64: return make_set<BW, 64>();
+  }
+  return KernelSet{};
+}
+
+KernelSet kernels_for(int bw, int bh) {
+  switch (bw) {
+  case 4: return set_for_bh<4>(bh);
+  case 8: return set_for_bh<8>(bh);
+  case 16: return set_for_bh<16>(bh);
+  case 32: return set_for_bh<32>(bh);
+  case 64: return set_for_bh<64>(bh);
+  }
+  return KernelSet{};
+}
+
+void set_err(char* err, size_t n, const std::string& s) {
+  if (err && n) {
+    std::snprintf(err, n, "%s", s.c_str());
+  }
+}
+
+} // namespace
+
+struct fft3d_plane {
+  fft3d::Params p{};
+  fft3d::Tables t;
+  fft3d_plane_desc d{};
+  int nb = 0, ncols = 0, nrows = 0, fbw = 0, bps = 4, pix = 0;
+  size_t specBytes = 0;
+  KernelSet ks;
+  fk::PlaneArgs args{};
+  hipStream_t own = nullptr;
+  std::string err;
+  size_t bytes = 0;
+
+  float* dTables = nullptr;  // windows, pattern, sharpen, degrid
+  void* dStage = nullptr;    // host-upload / host-readback staging (W*H*bps)
+  float* dBlocks = nullptr;  // windowed blocks awaiting overlap-add
+  float4* dKalman = nullptr; // bm == 0 state
+  bool kalmanFirst = true;
+
+  std::map<int, fk::c32*> ring;  // frame number -> spectrum
+  std::vector<fk::c32*> pool;    // retired spectra kept for reuse
+
+  ~fft3d_plane() {
+    for (auto& kv : ring) (void)hipFree(kv.second);
+    for (auto* m : pool) (void)hipFree(m);
+    (void)hipFree(dTables);
+    (void)hipFree(dStage);
+    (void)hipFree(dBlocks);
+    (void)hipFree(dKalman);
+    if (own) (void)hipStreamDestroy(own);
+    g_bytes -= bytes;
+  }
+
+  bool alloc(void** ptr, size_t n) {
+    if (hipMalloc(ptr, n) != hipSuccess) {
+      *ptr = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+    bytes += n;
+    g_bytes += n;
+    return true;
+  }
+
+  hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : own; }
+
+  fk::c32* spectrum(int fn) {
+    auto it = ring.find(fn);
+    if (it != ring.end()) return it->second;
+    fk::c32* m = nullptr;
+    if (!pool.empty()) {
+      m = pool.back();
+      pool.pop_back();
+    } else if (!alloc(reinterpret_cast<void**>(&m), specBytes)) {
+      return nullptr;
+    }
+    ring[fn] = m;
+    return m;
+  }
+
+  void retire(fk::c32* m) {
+    // keep at most bt+2 spare spectra (enough for sequential access); free the rest
+    if (pool.size() < (size_t)(p.bt + 2)) {
+      pool.push_back(m);
+    } else {
+      (void)hipFree(m);
+      bytes -= specBytes;
+      g_bytes -= specBytes;
+    }
+  }
+
+  int fail(int code, const std::string& why) {
+    err = why;
+    return code;
+  }
+
+  bool launched(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      err = std::string(what) + " launch failed: " + hipGetErrorString(e);
+      return false;
+    }
+    return true;
+  }
+};
+
+extern "C" {
+
+void fft3d_params_default(fft3d_params* p) {
+  fft3d::Script s;
+  p->sigma = s.sigma; p->beta = s.beta; p->bw = s.bw; p->bh = s.bh; p->bt = s.bt;
+  p->sharpen = s.sharpen; p->plane = s.plane; p->mode = s.mode; p->bordersize = s.bordersize;
+  p->precision = s.precision; p->degrid = s.degrid; p->scutoff = s.scutoff; p->svr = s.svr;
+  p->smin = s.smin; p->smax = s.smax; p->kratio = s.kratio; p->ow = s.ow; p->oh = s.oh;
+  p->wintype = s.wintype; p->interlaced = s.interlaced; p->sigma2 = s.sigma2;
+  p->sigma3 = s.sigma3; p->sigma4 = s.sigma4; p->oldfft = s.oldfft;
+}
+
+int fft3d_dev_init(int device, char* err, size_t errlen) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    set_err(err, errlen, "FFT3DCL: no HIP GPU device available");
+    return FFT3D_EDEVICE;
+  }
+  if (device < 0 || device >= n) {
+    set_err(err, errlen, "FFT3DCL: HIP device index out of range");
+    return FFT3D_EDEVICE;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    (void)hipGetLastError();
+    set_err(err, errlen, "FFT3DCL: hipSetDevice failed");
+    return FFT3D_EDEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    (void)hipGetLastError();
+    set_err(err, errlen, "FFT3DCL: hipGetDeviceProperties failed");
+    return FFT3D_EDEVICE;
+  }
+  return FFT3D_OK;
+}
+
+int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft3d_plane** out,
+                       char* err, size_t errlen) {
+  if (out) *out = nullptr;
+  if (!sp || !desc || !out) {
+    set_err(err, errlen, "FFT3DCL: null argument");
+    return FFT3D_EINVAL;
+  }
+  if (desc->width <= 0 || desc->height <= 0) {
+    set_err(err, errlen, "FFT3DCL: empty plane");
+    return FFT3D_EINVAL;
+  }
+  if (desc->sample_type == 1 ? desc->bits != 32 : (desc->bits < 8 || desc->bits > 16)) {
+    set_err(err, errlen, desc->sample_type == 1
+                             ? "FFT3DCL: only 32-bit float clips supported (16-bit float not ported yet)"
+                             : "FFT3DCL: integer bit depths 8-16 only");
+    return FFT3D_EINVAL;
+  }
+  fft3d::Script s;
+  s.sigma = sp->sigma; s.beta = sp->beta; s.bw = sp->bw; s.bh = sp->bh; s.bt = sp->bt;
+  s.sharpen = sp->sharpen; s.plane = sp->plane; s.mode = sp->mode; s.bordersize = sp->bordersize;
+  s.precision = sp->precision; s.degrid = sp->degrid; s.scutoff = sp->scutoff; s.svr = sp->svr;
+  s.smin = sp->smin; s.smax = sp->smax; s.kratio = sp->kratio; s.ow = sp->ow; s.oh = sp->oh;
+  s.wintype = sp->wintype; s.interlaced = sp->interlaced; s.sigma2 = sp->sigma2;
+  s.sigma3 = sp->sigma3; s.sigma4 = sp->sigma4; s.oldfft = sp->oldfft;
+
+  auto pl = new fft3d_plane();
+  pl->d = *desc;
+  std::string why = fft3d::validate(s, desc->width, desc->height, desc->bits, pl->p);
+  if (!why.empty()) {
+    delete pl;
+    set_err(err, errlen, why);
+    return FFT3D_EINVAL;
+  }
+  const fft3d::Params& p = pl->p;
+  pl->ks = kernels_for(p.bw, p.bh);
+  if (!pl->ks.analysis) {
+    delete pl;
+    set_err(err, errlen, "FFT3DCL: GPU setup failed: block too large for the GPU path (max 64x64)");
+    return FFT3D_EUNSUPPORTED;
+  }
+  ++g_live;
+  const int W = desc->width, H = desc->height, bw = p.bw, bh = p.bh;
+  fft3d::Grid grid = fft3d::Grid::of(p);
+  pl->nb = grid.nb;
+  pl->ncols = grid.ncols;
+  pl->nrows = grid.nrows;
+  pl->fbw = bw / 2 + 1;
+  pl->bps = desc->sample_type == 1 ? 4 : (desc->bits <= 8 ? 1 : 2);
+  pl->pix = desc->sample_type == 1 ? 0 : (desc->bits <= 8 ? 8 : 16);
+  pl->specBytes = sizeof(fk::c32) * (size_t)pl->nb * bh * (bw / 2);
+  pl->t = fft3d::buildTables(p);
+
+  float chromaNorm = 0.0f;
+  if (p.mode == 1 && desc->chroma) {
+    if (desc->sample_type == 1)
+      chromaNorm = 0.0f;
+    else if (desc->bits <= 8)
+      chromaNorm = 128.0f / 255.0f;
+    else
+      chromaNorm = 32768.0f / 65535.0f;
+  }
+
+  // one device table block: 3 windows, pattern, sharpen, degrid
+  const size_t nwin = (size_t)bw * bh, nbin = (size_t)pl->fbw * bh;
+  std::vector<float> host(3 * nwin + 4 * nbin, 0.0f);
+  std::copy(pl->t.winAna.begin(), pl->t.winAna.end(), host.begin());
+  std::copy(pl->t.winSynA.begin(), pl->t.winSynA.end(), host.begin() + nwin);
+  std::copy(pl->t.winSynB.begin(), pl->t.winSynB.end(), host.begin() + 2 * nwin);
+  std::copy(pl->t.pattern.begin(), pl->t.pattern.end(), host.begin() + 3 * nwin);
+  std::copy(pl->t.sharpen.begin(), pl->t.sharpen.end(), host.begin() + 3 * nwin + nbin);
+  std::copy(pl->t.degrid.begin(), pl->t.degrid.end(), host.begin() + 3 * nwin + 2 * nbin);
+
+  bool ok = hipStreamCreateWithFlags(&pl->own, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && pl->alloc(reinterpret_cast<void**>(&pl->dTables), host.size() * sizeof(float));
+  ok = ok && pl->alloc(&pl->dStage, (size_t)W * H * pl->bps);
+  ok = ok && pl->alloc(reinterpret_cast<void**>(&pl->dBlocks), sizeof(float) * (size_t)pl->nb * bw * bh);
+  if (ok && p.bm == 0)
+    ok = pl->alloc(reinterpret_cast<void**>(&pl->dKalman), sizeof(float4) * (size_t)pl->nb * nbin);
+  if (!ok) {
+    (void)hipGetLastError();
+    delete pl;
+    --g_live;
+    set_err(err, errlen, "FFT3DCL: GPU setup failed: GPU alloc failed");
+    return FFT3D_ENOMEM;
+  }
+  if (hipMemcpy(pl->dTables, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    delete pl;
+    --g_live;
+    set_err(err, errlen, "FFT3DCL: GPU setup failed: table upload failed");
+    return FFT3D_ELAUNCH;
+  }
+
+  fk::PlaneArgs& a = pl->args;
+  a.W = W;
+  a.H = H;
+  a.bw = bw;
+  a.bh = bh;
+  a.ow = p.ow;
+  a.oh = p.oh;
+  a.mode = p.mode;
+  a.nx = p.nx;
+  a.ny = p.ny;
+  a.ncols = pl->ncols;
+  a.nrows = pl->nrows;
+  a.nb = pl->nb;
+  a.pix = pl->pix;
+  a.inNorm = (pl->pix == 16) ? (float)(1u << (16 - desc->bits)) / 65535.0f : 1.0f;
+  a.invNorm = (pl->pix == 16) ? 1.0f / (float)(1u << (16 - desc->bits)) : 1.0f;
+  a.maxPix = (pl->pix == 16) ? (float)((1u << desc->bits) - 1) : 0.0f;
+  a.chromaNorm = chromaNorm;
+  a.bm = p.bm;
+  a.dg = p.degridOn ? 1 : 0;
+  a.pat = p.usePattern ? 1 : 0;
+  a.shp = p.sharpenOn ? 1 : 0;
+  a.sigmaX = p.beta * p.wienerS;
+  a.sigmaY = p.wienerS;
+  a.betaX = p.betaX;
+  a.beta = p.beta;
+  a.ssmin = p.ssmin;
+  a.ssmax = p.ssmax;
+  a.kalmanCN = p.kalmanCN;
+  a.kr2 = p.kratio * p.kratio;
+  a.winAna = pl->dTables;
+  a.winSynA = p.mode == 1 ? pl->dTables + nwin : pl->dTables;
+  a.winSynB = p.mode == 1 ? pl->dTables + 2 * nwin : pl->dTables;
+  a.pattern = pl->dTables + 3 * nwin;
+  a.sharpen = pl->dTables + 3 * nwin + nbin;
+  a.gs = reinterpret_cast<const fk::c32*>(pl->dTables + 3 * nwin + 2 * nbin);
+  *out = pl;
+  return FFT3D_OK;
+}
+
+void fft3d_plane_destroy(fft3d_plane* p) {
+  if (!p) return;
+  // work queued on caller streams may still read the plane's buffers
+  (void)hipDeviceSynchronize();
+  (void)hipGetLastError();
+  --g_live;
+  delete p;
+}
+
+const char* fft3d_plane_error(const fft3d_plane* p) { return p ? p->err.c_str() : ""; }
+
+int fft3d_plane_has(const fft3d_plane* p, int fn) { return p && p->ring.count(fn) ? 1 : 0; }
+
+int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, int src_on_device,
+                        void* stream) {
+  if (!p || !src) return FFT3D_EINVAL;
+  const size_t need = (size_t)p->d.width * p->d.height * p->bps;
+  if (bytes < need) return p->fail(FFT3D_EINVAL, "plane buffer too small");
+  hipStream_t s = p->stream(stream);
+  fk::c32* spec = p->spectrum(fn);
+  if (!spec) return p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed");
+  const void* plane = src;
+  if (!src_on_device) {
+    if (hipMemcpyAsync(p->dStage, src, need, hipMemcpyHostToDevice, s) != hipSuccess) {
+      (void)hipGetLastError();
+      p->ring.erase(fn);
+      p->retire(spec);
+      return p->fail(FFT3D_ELAUNCH, "plane upload failed");
+    }
+    plane = p->dStage;
+  }
+  const int grid = (p->nb + p->ks.G - 1) / p->ks.G;
+  AnalysisFn k = p->ks.analysis;
+  FK_LAUNCH(k, dim3(grid), dim3(64), s, p->args, plane, spec);
+  if (!p->launched("fft2d forward")) {
+    p->ring.erase(fn);
+    p->retire(spec);
+    return FFT3D_ELAUNCH;
+  }
+  return FFT3D_OK;
+}
+
+int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst, size_t bytes,
+                        int dst_on_device, void* stream) {
+  if (!p || !window || !dst) return FFT3D_EINVAL;
+  const int want = p->p.bm >= 2 ? p->p.bt : 1;
+  if (nwin != want) return p->fail(FFT3D_EINVAL, "window length does not match bt");
+  const size_t need = (size_t)p->d.width * p->d.height * p->bps;
+  if (bytes < need) return p->fail(FFT3D_EINVAL, "output buffer too small");
+  hipStream_t s = p->stream(stream);
+  fk::WindowArgs w{};
+  w.nwin = nwin;
+  w.cur = p->p.bm >= 2 ? p->p.current : 0;
+  for (int i = 0; i < 4; i++) {
+    const int fn = window[i < nwin ? i : 0];
+    auto it = p->ring.find(fn);
+    if (it == p->ring.end()) return p->fail(FFT3D_EMISSING, "missing spectra in window");
+    w.s[i] = it->second;
+  }
+  const int grid = (p->nb + p->ks.G - 1) / p->ks.G;
+  FilterFn kf = p->ks.filter_inverse;
+  const int kfirst = (p->p.bm == 0 && p->kalmanFirst) ? 1 : 0;
+  FK_LAUNCH(kf, dim3(grid), dim3(64), s, p->args, w, p->dKalman, kfirst, p->dBlocks);
+  if (!p->launched("filter/inverse")) return FFT3D_ELAUNCH;
+  if (p->p.bm == 0) p->kalmanFirst = false;
+  void* outp = dst_on_device ? dst : p->dStage;
+  SynthFn ksy = &fk::k_synthesize;
+  FK_LAUNCH(ksy, dim3((p->d.width + 15) / 16, (p->d.height + 15) / 16), dim3(256), s, p->args,
+            p->dBlocks, outp);
+  if (!p->launched("synthesize")) return FFT3D_ELAUNCH;
+  if (!dst_on_device) {
+    if (hipMemcpyAsync(dst, p->dStage, need, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      return p->fail(FFT3D_ELAUNCH, "readback failed");
+    }
+  }
+  return FFT3D_OK;
+}
+
+void fft3d_plane_prune(fft3d_plane* p, int lo, int hi) {
+  if (!p) return;
+  for (auto it = p->ring.begin(); it != p->ring.end();) {
+    const bool keep = (lo < 0 || it->first >= lo) && (hi < 0 || it->first <= hi);
+    if (keep) {
+      ++it;
+    } else {
+      p->retire(it->second);
+      it = p->ring.erase(it);
+    }
+  }
+}
+
+void fft3d_plane_clear(fft3d_plane* p) {
+  if (!p) return;
+  for (auto& kv : p->ring) p->retire(kv.second);
+  p->ring.clear();
+}
+
+int fft3d_live_instances(void) { return g_live.load(); }
+size_t fft3d_live_bytes(void) { return g_bytes.load(); }
+
+int fft3d_plane_get_info(const fft3d_plane* p, fft3d_plane_info* i) {
+  if (!p || !i) return FFT3D_EINVAL;
+  std::memset(i, 0, sizeof(*i));
+  i->nb = p->nb;
+  i->bw = p->p.bw;
+  i->bh = p->p.bh;
+  i->ow = p->p.ow;
+  i->oh = p->p.oh;
+  i->mode = p->p.mode;
+  i->fbw = p->fbw;
+  i->bm = p->p.bm;
+  i->bt = p->p.bt;
+  i->current = p->p.current;
+  i->ncols = p->ncols;
+  i->nrows = p->nrows;
+  i->blocks_per_wg = p->ks.G;
+  i->spectrum_bytes = p->specBytes;
+  i->device_bytes = p->bytes;
+  i->cached_frames = (int)p->ring.size();
+  return FFT3D_OK;
+}
+
+int fft3d_plane_get_table(const fft3d_plane* p, int which, float* dst) {
+  if (!p) return FFT3D_EINVAL;
+  const std::vector<float>* v = nullptr;
+  switch (which) {
+  case 0: v = &p->t.winAna; break;
+  case 1: v = &p->t.winSynA; break;
+  case 2: v = &p->t.winSynB; break;
+  case 3: v = &p->t.pattern; break;
+  case 4: v = &p->t.sharpen; break;
+  case 5: v = &p->t.degrid; break;
+  default: return FFT3D_EINVAL;
+  }
+  if (dst && !v->empty()) std::memcpy(dst, v->data(), v->size() * sizeof(float));
+  return (int)v->size();
+}
+
+int fft3d_plane_read_spectrum(fft3d_plane* p, int fn, float* dst) {
+  if (!p || !dst) return FFT3D_EINVAL;
+  auto it = p->ring.find(fn);
+  if (it == p->ring.end()) return p->fail(FFT3D_EMISSING, "missing spectra");
+  const int bw = p->p.bw, bh = p->p.bh, nc = bw / 2, fbw = p->fbw;
+  std::vector<fk::c32> h((size_t)p->nb * bh * nc);
+  if (hipStreamSynchronize(p->own) != hipSuccess ||
+      hipMemcpy(h.data(), it->second, p->specBytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    (void)hipGetLastError();
+    return p->fail(FFT3D_ELAUNCH, "spectrum readback failed");
+  }
+  for (int b = 0; b < p->nb; b++) {
+    const fk::c32* s = &h[(size_t)b * bh * nc];
+    float* d = dst + (size_t)b * bh * fbw * 2;
+    for (int r = 0; r < bh; r++) {
+      const int r2 = (bh - r) % bh;
+      for (int c = 1; c < nc; c++) {
+        d[2 * (r * fbw + c)] = s[r * nc + c].x;
+        d[2 * (r * fbw + c) + 1] = s[r * nc + c].y;
+      }
+      const fk::c32 wa = s[r * nc], wb = s[r2 * nc];
+      d[2 * (r * fbw)] = 0.5f * (wa.x + wb.x);
+      d[2 * (r * fbw) + 1] = 0.5f * (wa.y - wb.y);
+      d[2 * (r * fbw + nc)] = 0.5f * (wa.y + wb.y);
+      d[2 * (r * fbw + nc) + 1] = -0.5f * (wa.x - wb.x);
+    }
+  }
+  return FFT3D_OK;
+}
+
+} // extern "C"

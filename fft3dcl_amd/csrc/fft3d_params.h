@@ -1,0 +1,298 @@
+// Parameter contract and constant tables of the FFT3D denoise path (host side).
+//
+// Restates the reference's contract so the GPU path receives exactly the
+// same validated parameters and tables as its CPU backend:
+//   validate()          src/params.h:76-148 (roundPow2Clamp :62-72)
+//   window tables       src/cpu/tables.cpp:8-105, reference.cpp:47-58
+//   sigma pattern       src/cpu/tables.cpp:107-137
+//   sharpen LUT         src/cpu/tables.cpp:139-156
+//   degrid sample       src/cpu/reference.cpp:172-198 (computed here with an
+//                       FP64 DFT of the analysis window instead of FP32 FFT)
+// All tables are evaluated in double and stored as float, as the reference
+// does; float expressions keep the reference's operand order.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fft3d {
+
+// Script-level parameters (src/params.h:8-27), user units, reference defaults.
+struct Script {
+  float sigma = 2.0f, beta = 1.0f;
+  int bw = 32, bh = 32, bt = 3;
+  float sharpen = 0.0f;
+  int plane = 0, mode = 1, bordersize = 1, precision = 0;
+  float degrid = 1.0f, scutoff = 0.3f, svr = 1.0f, smin = 4.0f, smax = 20.0f, kratio = 2.0f;
+  int ow = 16, oh = 16, wintype = 0, interlaced = 0;
+  float sigma2 = 2.0f, sigma3 = 2.0f, sigma4 = 2.0f;
+  int oldfft = -1;
+};
+
+// Validated, derived parameters for one plane (src/params.h:30-59).
+struct Params {
+  float sigma, beta;
+  int bw, bh, bm, bt;
+  float sharpen;
+  bool sharpenOn;
+  int mode, bordersize, precision;
+  float degrid;
+  bool degridOn;
+  float scutoff, svr, smin, smax, kratio;
+  int ow, oh, wintype, interlaced;
+  float sigma2, sigma3, sigma4;
+  bool usePattern;
+  int current;
+  int nx, ny, totw, toth, oldfft;
+  float wienerS, betaX, kalmanCN, ssmin, ssmax;
+};
+
+inline int roundPow2Clamp(int v) {
+  unsigned logn = 0;
+  for (unsigned i = static_cast<unsigned>(v - 1); i > 0; i /= 2) logn++;
+  int r = 1 << logn;
+  return r < 4 ? 4 : (r > 512 ? 512 : r);
+}
+
+// Returns an empty string on success, else the reference's error text.
+inline std::string validate(const Script& s, int width, int height, int bits, Params& o) {
+  o = Params{};
+  o.sigma = s.sigma / 255.0f;
+  o.beta = s.beta;
+  o.bm = s.bt;
+  if (o.bm < -1 || o.bm > 4) return "FFT3DCL: valid modes for bt are -1,0,1,2,3,4";
+  o.bt = s.bt > 0 ? s.bt : 1;
+  o.sharpen = s.sharpen;
+  o.sharpenOn = s.sharpen != 0.0f;
+  o.mode = (s.mode < 0 || s.mode > 2) ? 0 : s.mode;
+  o.bordersize = s.bordersize;
+  o.degrid = s.degrid;
+  o.degridOn = s.degrid != 0.0f;
+  o.scutoff = s.scutoff;
+  o.svr = s.svr;
+  o.smin = s.smin / 255.0f;
+  o.smax = s.smax / 255.0f;
+  o.kratio = s.kratio;
+  o.wintype = (s.wintype < 0 || s.wintype > 2) ? 0 : s.wintype;
+  o.interlaced = s.interlaced;
+  o.sigma2 = s.sigma2;
+  o.sigma3 = s.sigma3;
+  o.sigma4 = s.sigma4;
+  o.usePattern = s.sigma != s.sigma2 || s.sigma3 != s.sigma4 || s.sigma != s.sigma3;
+  o.oldfft = s.oldfft;
+  o.current = o.bm == 1 ? 0 : (o.bm == 4 ? 2 : 1);
+  o.bw = roundPow2Clamp(s.bw);
+  o.bh = roundPow2Clamp(s.bh);
+  // ow/oh follow the block-size rounding; ow is forced even, both clamped to half a block.
+  o.ow = static_cast<int>(s.ow * (float)o.bw / (float)s.bw) & ~1;
+  o.oh = static_cast<int>(s.oh * (float)o.bh / (float)s.bh);
+  if (o.oh * 2 > o.bh) o.oh = o.bh / 2;
+  if (o.ow * 2 > o.bw) o.ow = o.bw / 2;
+  int precision = bits > 8 ? 2 : s.precision;
+  o.precision = (precision < 0 || precision > 2) ? 1 : precision;
+  if (o.mode == 2) return "FFT3DCL: mode 2 is not implemented yet in this port (use mode 0 or 1)";
+  if (o.interlaced) return "FFT3DCL: interlaced is not implemented yet in this port";
+  if (o.mode == 0 || (o.bw == o.ow * 2 && o.bh == o.oh * 2)) {
+    o.ny = static_cast<int>((height + o.bh * 1.5 - 1) / o.bh);
+    o.nx = static_cast<int>((width + o.bw * 1.5 - 1) / o.bw);
+  } else {
+    o.nx = ((width + o.ow - 1) / (o.bw - o.ow) + 2) / 2;
+    o.ny = ((height + o.oh - 1) / (o.bh - o.oh) + 2) / 2;
+  }
+  o.totw = o.nx * o.bw;
+  o.toth = o.ny * o.bh;
+  o.wienerS = o.sigma * o.sigma * o.bt * o.bw * o.bh;
+  o.betaX = (o.beta - 1.0f) / o.beta;
+  o.kalmanCN = o.sigma * o.sigma * o.bw * o.bh;
+  o.ssmin = o.smin * o.smin * o.bw * o.bh * o.bt;
+  o.ssmax = o.smax * o.smax * o.bw * o.bh * o.bt;
+  return {};
+}
+
+// Block grid of one plane (src/cpu/reference.cpp:23-46). Mode 1: ncols x nrows
+// blocks at (-ow + n*(bw-ow), -oh + m*(bh-oh)), index m*ncols+n, parity n&1.
+// Mode 0: grid U (nx*ny at (i*bw, j*bh)) then grid S shifted by -bw/2,-bh/2.
+struct Grid {
+  int ncols = 0, nrows = 0, nb = 0;
+  static Grid of(const Params& p) {
+    Grid g;
+    if (p.mode == 0) {
+      g.ncols = p.nx;
+      g.nrows = 2 * p.ny;
+      g.nb = 2 * p.nx * p.ny;
+    } else {
+      g.ncols = 2 * p.nx;
+      g.nrows = 2 * p.ny;
+      g.nb = g.ncols * g.nrows;
+    }
+    return g;
+  }
+};
+
+struct Tables {
+  std::vector<float> winAna, winSynA, winSynB; // bw*bh, row-major
+  std::vector<float> pattern;                   // fbw*bh, empty when off
+  std::vector<float> sharpen;                   // fbw*bh, empty when off
+  std::vector<float> degrid;                    // 2*fbw*bh (re,im), empty when off
+};
+
+namespace detail {
+constexpr double kPi = 3.14159265358979323846;
+
+// Half-cosine ramps of the mode-1 windows: analysis an[i], synthesis sy[i], i < o.
+inline void ramps(int o, int wintype, std::vector<double>& an, std::vector<double>& sy) {
+  an.assign(o, 1.0);
+  sy.assign(o, 1.0);
+  for (int i = 0; i < o; i++) {
+    double c = std::cos(kPi * (i - o + 0.5) / (o * 2.0));
+    if (wintype == 1) {
+      an[i] = std::sqrt(c);
+      sy[i] = an[i] * an[i] * an[i];
+    } else if (wintype == 2) {
+      an[i] = 1.0;
+      sy[i] = c * c;
+    } else {
+      an[i] = c;
+      sy[i] = c;
+    }
+  }
+}
+
+// Separable 1-D profile: rising ramp, flat middle, falling ramp.
+inline std::vector<double> profile(int n, int o, const std::vector<double>& r, bool reversed) {
+  std::vector<double> w(n, 1.0);
+  for (int l = 0; l < o; l++) w[l] = reversed ? r[o - 1 - l] : r[l];
+  for (int l = n - o; l < n; l++) {
+    int x = l - (n - o);
+    w[l] = reversed ? r[x] : r[o - 1 - x];
+  }
+  return w;
+}
+} // namespace detail
+
+inline Tables buildTables(const Params& p) {
+  using namespace detail;
+  const int bw = p.bw, bh = p.bh, fbw = bw / 2 + 1;
+  Tables t;
+  t.winAna.resize(bw * bh);
+  t.winSynA.resize(bw * bh);
+  t.winSynB.resize(bw * bh);
+  if (p.mode == 0) {
+    // sqrt(0.5 (cos^2 x + cos^2 y)) over pixel centres (tables.cpp:8-19).
+    for (int ly = 0; ly < bh; ly++) {
+      double cy = std::cos((ly - bh / 2.0 + 0.5) * kPi / bh);
+      for (int lx = 0; lx < bw; lx++) {
+        double cx = std::cos((lx - bw / 2.0 + 0.5) * kPi / bw);
+        t.winAna[ly * bw + lx] = (float)std::sqrt(0.5 * (cx * cx + cy * cy));
+      }
+    }
+    t.winSynA = t.winAna;
+    t.winSynB = t.winAna;
+  } else {
+    std::vector<double> anx, syx, any, syy;
+    ramps(p.ow, p.wintype, anx, syx);
+    ramps(p.oh, p.wintype, any, syy);
+    std::vector<double> wxA = profile(bw, p.ow, anx, false);
+    std::vector<double> wxSA = profile(bw, p.ow, syx, false);
+    std::vector<double> wxSB = profile(bw, p.ow, syx, true);
+    std::vector<double> wyA = profile(bh, p.oh, any, false);
+    std::vector<double> wySA = profile(bh, p.oh, syy, false);
+    for (int ly = 0; ly < bh; ly++)
+      for (int lx = 0; lx < bw; lx++) {
+        t.winAna[ly * bw + lx] = (float)(wxA[lx] * wyA[ly]);
+        t.winSynA[ly * bw + lx] = (float)(wxSA[lx] * wySA[ly]);
+        // odd-parity synthesis: reversed-ramp profile sampled (bw-ow) to the right
+        int sx = (lx + bw - p.ow) % bw;
+        t.winSynB[ly * bw + lx] = (float)(wxSB[sx] * wySA[ly]);
+      }
+  }
+  if (p.usePattern) {
+    auto scale = [&](float s) {
+      double v = s / 255.0;
+      return (float)(v * v * p.bt * bw * bh);
+    };
+    const float s1 = scale(p.sigma * 255.0f), s2 = scale(p.sigma2), s3 = scale(p.sigma3),
+                s4 = scale(p.sigma4);
+    const float ft2 = (float)(std::sqrt(0.5) / 2.0), ft3 = (float)(std::sqrt(0.5) / 4.0);
+    t.pattern.resize(fbw * bh);
+    for (int h = 0; h < bh; h++) {
+      float fy = (bh - 2.0f * std::fabs(h - bh / 2.0f)) / bh;
+      for (int w = 0; w < fbw; w++) {
+        float fx = w * 1.0f / fbw;
+        float f = std::sqrt((fx * fx + fy * fy) * 0.5f);
+        float v;
+        if (f < ft3)
+          v = s4 + (s3 - s4) * f / ft3;
+        else if (f < ft2)
+          v = s3 + (s2 - s3) * (f - ft3) / (ft2 - ft3);
+        else
+          v = s1 + (s2 - s1) * (1.0f - f) / (1.0f - ft2);
+        t.pattern[h * fbw + w] = v;
+      }
+    }
+  }
+  if (p.sharpenOn) {
+    t.sharpen.resize(fbw * bh);
+    for (int j = 0; j < bh; j++) {
+      int dj = j < bh / 2 ? j : bh - j;
+      float d2v = (float)(dj * dj) * (p.svr * p.svr) / ((bh / 2.0f) * (bh / 2.0f));
+      for (int i = 0; i < fbw; i++) {
+        float d2 = d2v + (float)(i * i) / ((bw / 2.0f) * (bw / 2.0f));
+        t.sharpen[j * fbw + i] = p.sharpen * (1.0f - std::exp(-d2 / (2.0f * p.scutoff * p.scutoff)));
+      }
+    }
+  }
+  if (p.degridOn) {
+    // gs = DFT2(winAna) / DFT2(winAna)[0,0].re * degrid, FP64 direct DFT.
+    std::vector<double> cr(bw * bw), ci(bw * bw), rr(bh * bh), ri(bh * bh);
+    for (int a = 0; a < bw; a++)
+      for (int x = 0; x < bw; x++) {
+        double ang = -2.0 * kPi * ((long)a * x % bw) / bw;
+        cr[a * bw + x] = std::cos(ang);
+        ci[a * bw + x] = std::sin(ang);
+      }
+    for (int a = 0; a < bh; a++)
+      for (int y = 0; y < bh; y++) {
+        double ang = -2.0 * kPi * ((long)a * y % bh) / bh;
+        rr[a * bh + y] = std::cos(ang);
+        ri[a * bh + y] = std::sin(ang);
+      }
+    // rows: R[y][c] = sum_x w[y][x] e^{-2pi i c x/bw}
+    std::vector<double> Rr(bh * fbw), Ri(bh * fbw);
+    for (int y = 0; y < bh; y++)
+      for (int c = 0; c < fbw; c++) {
+        double sr = 0, si = 0;
+        for (int x = 0; x < bw; x++) {
+          double w = t.winAna[y * bw + x];
+          sr += w * cr[c * bw + x];
+          si += w * ci[c * bw + x];
+        }
+        Rr[y * fbw + c] = sr;
+        Ri[y * fbw + c] = si;
+      }
+    std::vector<double> Sr(bh * fbw), Si(bh * fbw);
+    for (int r = 0; r < bh; r++)
+      for (int c = 0; c < fbw; c++) {
+        double sr = 0, si = 0;
+        for (int y = 0; y < bh; y++) {
+          double a = Rr[y * fbw + c], b = Ri[y * fbw + c];
+          double e = rr[r * bh + y], f = ri[r * bh + y];
+          sr += a * e - b * f;
+          si += a * f + b * e;
+        }
+        Sr[r * fbw + c] = sr;
+        Si[r * fbw + c] = si;
+      }
+    const double dc = Sr[0];
+    t.degrid.resize(2 * fbw * bh);
+    for (int i = 0; i < fbw * bh; i++) {
+      t.degrid[2 * i] = (float)(Sr[i] / dc * p.degrid);
+      t.degrid[2 * i + 1] = (float)(Si[i] / dc * p.degrid);
+    }
+  }
+  return t;
+}
+
+} // namespace fft3d

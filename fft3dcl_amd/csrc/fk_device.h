@@ -1,0 +1,50 @@
+// Device-side basics shared by the FFT3D kernels.
+//
+// The kernels are plain HIP for gfx950. The same source also compiles as host
+// C++ against a thread-per-lane emulation of the few HIP facilities they use
+// (FFT3D_CPU_EMU, tests/emu/hip_emu.h), which is how the index arithmetic,
+// LDS layouts and barriers are checked on a machine without a GPU.
+#pragma once
+
+#if defined(FFT3D_CPU_EMU)
+#include "hip_emu.h"
+#else
+#include <hip/hip_runtime.h>
+// Launch through a kernel function pointer (template instantiations are
+// selected at run time from a dispatch table).
+#define FK_LAUNCH(fn, grid, block, stream, ...) \
+  fn<<<(grid), (block), 0, (stream)>>>(__VA_ARGS__)
+#endif
+
+#include <cstdint>
+
+namespace fk {
+
+struct c32 {
+  float x, y;
+};
+
+__device__ __forceinline__ c32 cmake(float x, float y) { return c32{x, y}; }
+__device__ __forceinline__ c32 cadd(c32 a, c32 b) { return c32{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ c32 csub(c32 a, c32 b) { return c32{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ c32 cconj(c32 a) { return c32{a.x, -a.y}; }
+__device__ __forceinline__ c32 cscale(c32 a, float s) { return c32{a.x * s, a.y * s}; }
+
+// Reciprocal and square root used by the filter. Tolerances of the path are
+// far above one ulp, so the hardware approximations (1 ulp) are used.
+__device__ __forceinline__ float frcp(float x) {
+#if defined(FFT3D_CPU_EMU)
+  return 1.0f / x;
+#else
+  return __builtin_amdgcn_rcpf(x);
+#endif
+}
+__device__ __forceinline__ float fsqrt(float x) {
+#if defined(FFT3D_CPU_EMU)
+  return emu_sqrtf(x);
+#else
+  return __builtin_sqrtf(x);
+#endif
+}
+
+} // namespace fk

@@ -1,0 +1,520 @@
+// HIP kernels of the FFT3D denoise path (gfx950, wave64).
+//
+// Data layout in HBM
+//   plane    : W x H samples, packed rows (u8, u16 or f32), read in place.
+//   spectrum : per frame, per block b, bh rows x bw/2 packed columns of
+//              complex float ([b][r][c], c fastest). Column c >= 1 holds the
+//              2-D DFT bin (r, c); column 0 holds W[r] = X[r][0] + i X[r][bw/2],
+//              the two real-signal columns packed into one complex column
+//              (both are Hermitian in r, so the packing is exact). This is
+//              bw*bh*4 bytes per block instead of the reference's
+//              (bw/2+1)*bh*8 (src/cpu/reference.cpp:161-167).
+//   kalman   : per bin (b, r, col<bw/2+1) float4 {last.re, last.im, covar, proc}.
+//
+// Work mapping. A 64-lane workgroup owns G = 64 / max(bh/2, bw/2) blocks. In
+// a row pass lane j of block g transforms the packed row pair (2j, 2j+1) as
+// one complex sequence; in a column pass lane c transforms column c (c = 0 is
+// the packed DC/Nyquist column). Each 1-D transform runs inside one lane's
+// registers (fk_fft.h); rows and columns meet through one padded LDS
+// transpose per direction.
+//
+// Reference kernels this replaces (src/cl/kernels.h): fft2d_fwd :141-176,
+// filter :185-345, kalman_init :348-363 (folded into the filter), fft2d_inv
+// :368-404, synthesize :409-448.
+#pragma once
+
+#include "fk_fft.h"
+
+namespace fk {
+
+struct PlaneArgs {
+  int W, H;            // plane size in samples
+  int bw, bh, ow, oh;  // block size and overlap
+  int mode;            // 0: two 1:1 grids, 1: overlapped grid
+  int nx, ny;          // grid cells (src/params.h:131-139)
+  int ncols, nrows;    // mode 1 block columns / rows
+  int nb;              // blocks per frame
+  int pix;             // 0 float samples, 8 u8, 16 u16
+  float inNorm;        // u16 -> float factor 2^(16-bits)/65535
+  float invNorm;       // float -> u16 factor 1/2^(16-bits)
+  float maxPix;        // 2^bits - 1
+  float chromaNorm;    // chroma centring (0 when off)
+  int bm;              // temporal mode -1..4
+  int dg, pat, shp;    // degrid, sigma pattern, sharpen on/off
+  float sigmaX, sigmaY, betaX, beta;
+  float ssmin, ssmax, kalmanCN, kr2;
+  const float* winAna;  // bw*bh analysis window
+  const float* winSynA; // bw*bh synthesis window, even block columns (mode 0: winAna)
+  const float* winSynB; // bw*bh synthesis window, odd block columns
+  const float* pattern; // (bw/2+1)*bh sigma pattern (pat)
+  const float* sharpen; // (bw/2+1)*bh sharpen gains (shp)
+  const c32* gs;        // (bw/2+1)*bh degrid sample (dg)
+};
+
+// The temporal window handed to the filter: frames in temporal order.
+struct WindowArgs {
+  const c32* s[4];
+  int nwin;  // bt for bm >= 2, else 1
+  int cur;   // index of the current frame in s[]
+};
+
+template <int BW, int BH>
+struct Shape {
+  static constexpr int NROW = BH / 2;  // packed row pairs per block
+  static constexpr int NCOL = BW / 2;  // stored columns per block
+  static constexpr int LPB = NROW > NCOL ? NROW : NCOL;
+  static constexpr int G = 64 / LPB;   // blocks per workgroup
+  static constexpr int FBW = BW / 2 + 1;
+  static constexpr int BSTR = BH * NCOL;  // complex values stored per block
+  static constexpr int ZP = BW + 1;       // LDS pitch, row-pass buffer
+  static constexpr int YP = BH + 1;       // LDS pitch, column-pass buffer
+  static constexpr int LDS_C =
+      (G * NROW * ZP > G * FBW * YP) ? G * NROW * ZP : G * FBW * YP;
+};
+
+// Direct3D MIRROR addressing (src/cpu/reference.cpp:10-17).
+__device__ __forceinline__ int mirror(int x, int w) {
+  if (x >= 0 && x < w) return x;
+  int p = x % (2 * w);
+  if (p < 0) p += 2 * w;
+  if (p >= w) p = 2 * w - 1 - p;
+  return p;
+}
+
+// Block origin and synthesis parity (src/cpu/reference.cpp:23-46).
+__device__ __forceinline__ void block_origin(const PlaneArgs& P, int b, int& x0, int& y0,
+                                             int& par) {
+  if (P.mode == 1) {
+    const int m = b / P.ncols, n = b - m * P.ncols;
+    x0 = -P.ow + n * (P.bw - P.ow);
+    y0 = -P.oh + m * (P.bh - P.oh);
+    par = n & 1;
+  } else {
+    const int per = P.nx * P.ny;
+    const int grid = b >= per ? 1 : 0;
+    const int r = b - grid * per, j = r / P.nx, i = r - j * P.nx;
+    x0 = i * P.bw - (grid ? P.bw / 2 : 0);
+    y0 = j * P.bh - (grid ? P.bh / 2 : 0);
+    par = 0;
+  }
+}
+
+// Sample -> float (src/cpu/reference.cpp:82-105; src/cl/kernels.h:22-34).
+__device__ __forceinline__ float fetch(const PlaneArgs& P, const void* plane, int x, int y) {
+  const size_t i = (size_t)mirror(y, P.H) * P.W + mirror(x, P.W);
+  if (P.pix == 8) return (float)static_cast<const uint8_t*>(plane)[i] * (1.0f / 255.0f);
+  if (P.pix == 16) return (float)static_cast<const uint16_t*>(plane)[i] * P.inNorm;
+  return static_cast<const float*>(plane)[i];
+}
+
+// --------------------------------------------------------------- analysis
+
+// Windowed 2-D R2C FFT of block b. On return, lanes j < NCOL hold packed
+// column j of the spectrum in (yr[r], yi[r]), r < BH. All lanes must call.
+template <int BW, int BH>
+__device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* plane, int b,
+                                              bool live, int g, int j, c32* lds,
+                                              float (&yr)[BH], float (&yi)[BH]) {
+  using S = Shape<BW, BH>;
+  if (live && j < S::NROW) {
+    int x0, y0, par;
+    block_origin(P, b, x0, y0, par);
+    float zr[BW], zi[BW];
+    const int r0 = 2 * j;
+    const float* w0 = P.winAna + r0 * BW;
+    const float* w1 = w0 + BW;
+#pragma unroll
+    for (int k = 0; k < BW; k++) {
+      zr[k] = (fetch(P, plane, x0 + k, y0 + r0) - P.chromaNorm) * w0[k];
+      zi[k] = (fetch(P, plane, x0 + k, y0 + r0 + 1) - P.chromaNorm) * w1[k];
+    }
+    fft_reg<BW, false>(zr, zi);
+    c32* Z = lds + (g * S::NROW + j) * S::ZP;
+#pragma unroll
+    for (int k = 0; k < BW; k++) Z[k] = cmake(zr[k], zi[k]);
+  }
+  __syncthreads();
+  if (live && j < S::NCOL) {
+    const int c = j;
+    const c32* Zg = lds + g * S::NROW * S::ZP;
+#pragma unroll
+    for (int p = 0; p < S::NROW; p++) {
+      const c32* Z = Zg + p * S::ZP;
+      if (c == 0) {
+        // rows 2p, 2p+1 are real at DC and Nyquist: X = (Re Z, Im Z)
+        const c32 a = Z[0], h = Z[BW / 2];
+        yr[2 * p] = a.x;
+        yi[2 * p] = h.x;
+        yr[2 * p + 1] = a.y;
+        yi[2 * p + 1] = h.y;
+      } else {
+        // X_2p[c] = (Z[c] + conj Z[-c]) / 2, X_2p+1[c] = (Z[c] - conj Z[-c]) / 2i
+        const c32 zc = Z[c], zm = Z[BW - c];
+        yr[2 * p] = 0.5f * (zc.x + zm.x);
+        yi[2 * p] = 0.5f * (zc.y - zm.y);
+        yr[2 * p + 1] = 0.5f * (zc.y + zm.y);
+        yi[2 * p + 1] = -0.5f * (zc.x - zm.x);
+      }
+    }
+    fft_reg<BH, false>(yr, yi);
+  }
+}
+
+template <int BW, int BH>
+__global__ __launch_bounds__(64) void k_analysis(PlaneArgs P, const void* plane, c32* spec) {
+  using S = Shape<BW, BH>;
+  __shared__ c32 lds[S::LDS_C];
+  const int lane = threadIdx.x, g = lane / S::LPB, j = lane - g * S::LPB;
+  const int b = blockIdx.x * S::G + g;
+  const bool live = b < P.nb;
+  float yr[BH], yi[BH];
+  forward_block<BW, BH>(P, plane, b, live, g, j, lds, yr, yi);
+  if (live && j < S::NCOL) {
+    c32* d = spec + (size_t)b * S::BSTR + j;
+#pragma unroll
+    for (int r = 0; r < BH; r++) d[r * S::NCOL] = cmake(yr[r], yi[r]);
+  }
+}
+
+// ---------------------------------------------------------------- filter
+
+// Wiener gain (src/cpu/reference.cpp:201-209). The reference's test
+// 1/sigmaX > 1/psd is evaluated as psd > thr with thr = wiener_threshold(sX)
+// (no reciprocal per call); the gain is continuous at the threshold.
+__device__ __forceinline__ float wiener_threshold(float sX) {
+  if (sX > 0.0f) return sX;
+  // 1/(+0) = +inf passes every psd; 1/(-0) and 1/negative never pass.
+  return (sX == 0.0f && copysignf(1.0f, sX) > 0.0f) ? -1.0f : __builtin_inff();
+}
+__device__ __forceinline__ c32 wiener(c32 v, float thr, float sY, float betaX) {
+  const float psd = v.x * v.x + v.y * v.y + 1e-37f;
+  const float mul = psd > thr ? 1.0f - sY * frcp(psd) : betaX;
+  return cscale(v, mul);
+}
+
+// Temporal filter + sharpen of one bin (src/cpu/reference.cpp:246-409,
+// src/cl/kernels.h:206-342). v[t] are the bin's values in temporal order,
+// d the degrid correction, ks the Kalman state of the bin (bm == 0).
+__device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4], c32 d,
+                                          float sX, float sY, float lut, float kcn, float4* ks,
+                                          bool kfirst) {
+  c32 f;
+  sX = wiener_threshold(sX);
+  if (P.bm == 2) {
+    const c32 pv = v[0], cu = v[1];
+    c32 a = csub(cu, pv), bb = cadd(cu, pv);
+    if (P.dg) bb = csub(bb, cscale(d, 2.0f));
+    a = wiener(a, sX, sY, P.betaX);
+    bb = wiener(bb, sX, sY, P.betaX);
+    f = cscale(cadd(a, bb), 0.5f);
+    if (P.dg) f = cadd(f, d);
+  } else if (P.bm == 3) {
+    const c32 pv = v[0], cu = v[1], nx = v[2];
+    const float K = 0.8660254f;
+    const c32 t1 = cadd(pv, nx);
+    const c32 y1 = csub(cu, cscale(t1, 0.5f));
+    const c32 dn = csub(nx, pv);
+    const c32 y2 = cmake(K * dn.y, -K * dn.x);
+    c32 d0 = cadd(y1, y2), d1 = cadd(cu, t1), d2 = csub(y1, y2);
+    if (P.dg) d1 = csub(d1, cscale(d, 3.0f));
+    d0 = wiener(d0, sX, sY, P.betaX);
+    d1 = wiener(d1, sX, sY, P.betaX);
+    d2 = wiener(d2, sX, sY, P.betaX);
+    f = cscale(cadd(cadd(d0, d1), d2), 1.0f / 3.0f);
+    if (P.dg) f = cadd(f, d);
+  } else if (P.bm == 4) {
+    const c32 p2 = v[0], p1 = v[1], cu = v[2], nx = v[3];
+    // 4-point temporal DFT as two radix-2 stages
+    const c32 s02 = cadd(p2, cu), d02 = csub(cu, p2);
+    const c32 s13 = cadd(p1, nx), d13 = csub(nx, p1);
+    c32 fc = cadd(s02, s13), fp2 = csub(s02, s13);
+    c32 fp1 = cmake(d02.x - d13.y, d02.y + d13.x);  // (c - p2) + i (n - p1)
+    c32 fn = cmake(d02.x + d13.y, d02.y - d13.x);   // (c - p2) - i (n - p1)
+    if (P.dg) fp1 = csub(fp1, cscale(d, 4.0f));
+    fp2 = wiener(fp2, sX, sY, P.betaX);
+    fp1 = wiener(fp1, sX, sY, P.betaX);
+    fc = wiener(fc, sX, sY, P.betaX);
+    fn = wiener(fn, sX, sY, P.betaX);
+    f = cscale(cadd(cadd(cadd(fp2, fp1), fc), fn), 0.25f);
+    if (P.dg) f = cadd(f, d);
+  } else if (P.bm == 1) {
+    c32 x = v[0];
+    if (P.dg) x = csub(x, d);
+    x = wiener(x, sX, sY, P.betaX);
+    f = P.dg ? cadd(x, d) : x;
+  } else if (P.bm == 0) {
+    // Kalman (src/cpu/reference.cpp:334-383); first frame initialises state.
+    const c32 cu = v[0];
+    float4 st = kfirst ? make_float4(cu.x, cu.y, kcn, kcn) : *ks;
+    const float dr = cu.x - st.x, di = cu.y - st.y;
+    const float mask = fmaxf(dr * dr, di * di);
+    const float s = st.z + st.w;
+    const float gain = s / (s + kcn);
+    float np = gain * gain * kcn, nc = (1.0f - gain) * s;
+    float fr = gain * cu.x + (1.0f - gain) * st.x;
+    float fi = gain * cu.y + (1.0f - gain) * st.y;
+    if (mask > P.kr2 * kcn) {
+      fr = cu.x;
+      fi = cu.y;
+      nc = kcn;
+      np = kcn;
+    }
+    *ks = make_float4(fr, fi, nc, np);
+    f = cmake(fr, fi);
+  } else {
+    f = v[0];  // bm == -1: the current frame passes through
+  }
+  if (P.shp) {
+    c32 x = P.dg ? csub(f, d) : f;
+    const float psd = x.x * x.x + x.y * x.y + 1e-37f;
+    const float q = psd * P.ssmax * frcp((psd + P.ssmin) * (psd + P.ssmax));
+    const float gsh = 1.0f + lut * fsqrt(q);
+    x = cscale(x, gsh);
+    f = P.dg ? cadd(x, d) : x;
+  }
+  return f;
+}
+
+// Per-bin constants of (row r, column col < bw/2+1).
+struct BinConst {
+  float sX, sY, lut, kcn;
+  c32 gsv;
+};
+template <int BW>
+__device__ __forceinline__ BinConst bin_const(const PlaneArgs& P, int r, int col) {
+  constexpr int FBW = BW / 2 + 1;
+  BinConst k;
+  const int i = r * FBW + col;
+  if (P.pat) {
+    const float pv = P.pattern[i];
+    k.sX = pv * P.beta;
+    k.sY = pv;
+    k.kcn = pv;
+  } else {
+    k.sX = P.sigmaX;
+    k.sY = P.sigmaY;
+    k.kcn = P.kalmanCN;
+  }
+  k.lut = P.shp ? P.sharpen[i] : 0.0f;
+  k.gsv = P.dg ? P.gs[i] : cmake(0.0f, 0.0f);
+  return k;
+}
+
+// Temporal filter of every bin of block b into the LDS column buffer
+// Yg[col * YP + r] (col < bw/2+1, unpacked). The BH*(BW/2+1) bins form one
+// virtual list, walked with stride LPB by the block's lanes, so every lane
+// filters the same number of bins: index k < BH*NCOL is stored bin
+// (k / NCOL, k % NCOL), with column 0 the DC column unpacked from the packed
+// pair; the BH remaining indices are the Nyquist column. All lanes of the
+// block call this with lb = lane index within the block.
+template <int BW, int BH>
+__device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArgs& Wn, int b,
+                                             int lb, c32* Yg, float4* kstate, bool kfirst) {
+  using S = Shape<BW, BH>;
+  constexpr int FBW = S::FBW, NCOL = S::NCOL, T = BH * FBW;
+  const size_t base = (size_t)b * S::BSTR;
+  const int nwin = Wn.nwin;
+  // block DC of the current frame: X[0][0] = Re W[0]
+  const float dc = P.dg ? Wn.s[Wn.cur][base].x : 0.0f;
+  float4* kb = kstate ? kstate + (size_t)b * T : nullptr;
+#pragma unroll 1
+  for (int k = lb; k < T; k += S::LPB) {
+    int r, col;
+    if (k < BH * NCOL) {
+      r = k / NCOL;
+      col = k - r * NCOL;
+    } else {
+      r = k - BH * NCOL;
+      col = NCOL;
+    }
+    const int r2 = (BH - r) & (BH - 1);
+    c32 v[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      v[t] = cmake(0.0f, 0.0f);
+      if (t < nwin) {
+        if (col != 0 && col != NCOL) {
+          v[t] = Wn.s[t][base + r * NCOL + col];
+        } else {
+          // X[r][0] = (W[r] + conj W[-r]) / 2, X[r][h] = (W[r] - conj W[-r]) / 2i
+          const c32 wa = Wn.s[t][base + r * NCOL], wb = Wn.s[t][base + r2 * NCOL];
+          v[t] = col == 0 ? cmake(0.5f * (wa.x + wb.x), 0.5f * (wa.y - wb.y))
+                          : cmake(0.5f * (wa.y + wb.y), -0.5f * (wa.x - wb.x));
+        }
+      }
+    }
+    const BinConst kc = bin_const<BW>(P, r, col);
+    Yg[col * S::YP + r] = filter_bin(P, v, cscale(kc.gsv, dc), kc.sX, kc.sY, kc.lut,
+                                     kc.kcn, kb ? kb + r * FBW + col : nullptr, kfirst);
+  }
+}
+
+// Packs the filtered DC and Nyquist columns back into column 0 as
+// W'[r] = H0[r] + i Hh[r], H = (F[r] + conj F[-r]) / 2 the Hermitian part
+// (the only part a real inverse transform keeps). All lanes call.
+template <int BW, int BH>
+__device__ __forceinline__ void repack_block(bool live, int lb, c32* Yg) {
+  using S = Shape<BW, BH>;
+  constexpr int NT = (BH + S::LPB - 1) / S::LPB;
+  c32 w[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int r = t * S::LPB + lb;
+    w[t] = cmake(0.0f, 0.0f);
+    if (live && r < BH) {
+      const int r2 = (BH - r) & (BH - 1);
+      const c32 f0 = Yg[r], g0 = Yg[r2];
+      const c32 fh = Yg[S::NCOL * S::YP + r], gh = Yg[S::NCOL * S::YP + r2];
+      const c32 h0 = cscale(cadd(f0, cconj(g0)), 0.5f);
+      const c32 hh = cscale(cadd(fh, cconj(gh)), 0.5f);
+      w[t] = cmake(h0.x - hh.y, h0.y + hh.x);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int r = t * S::LPB + lb;
+    if (live && r < BH) Yg[r] = w[t];
+  }
+  __syncthreads();
+}
+
+// Inverse 2-D C2R of block columns 0..NCOL-1 held in the LDS column buffer
+// (column 0 packed). On return lanes j < NROW hold real rows 2j (xr) and
+// 2j+1 (xi), scaled by 1/(bw*bh). All lanes call.
+template <int BW, int BH>
+__device__ __forceinline__ void inverse_block(bool live, int j, c32* Yg, float (&xr)[BW],
+                                              float (&xi)[BW]) {
+  using S = Shape<BW, BH>;
+  if (live && j < S::NCOL) {
+    float yr[BH], yi[BH];
+    c32* Y = Yg + j * S::YP;
+#pragma unroll
+    for (int r = 0; r < BH; r++) {
+      yr[r] = Y[r].x;
+      yi[r] = Y[r].y;
+    }
+    fft_reg<BH, true>(yr, yi);
+#pragma unroll
+    for (int r = 0; r < BH; r++) Y[r] = cmake(yr[r], yi[r]);
+  }
+  __syncthreads();
+  if (live && j < S::NROW) {
+    const int p = j;
+    {
+      const c32 v = Yg[2 * p], u = Yg[2 * p + 1];
+      xr[0] = v.x;
+      xi[0] = u.x;
+      xr[BW / 2] = v.y;
+      xi[BW / 2] = u.y;
+    }
+#pragma unroll
+    for (int c = 1; c < S::NCOL; c++) {
+      const c32 a = Yg[c * S::YP + 2 * p], q = Yg[c * S::YP + 2 * p + 1];
+      xr[c] = a.x - q.y;
+      xi[c] = a.y + q.x;
+      xr[BW - c] = a.x + q.y;
+      xi[BW - c] = -a.y + q.x;
+    }
+    fft_reg<BW, true>(xr, xi);
+    const float sc = 1.0f / (float)(BW * BH);
+#pragma unroll
+    for (int k = 0; k < BW; k++) {
+      xr[k] *= sc;
+      xi[k] *= sc;
+    }
+  }
+}
+
+// Filter + inverse + synthesis window; writes windowed blocks [b][ly][lx].
+template <int BW, int BH>
+__global__ __launch_bounds__(64) void k_filter_inverse(PlaneArgs P, WindowArgs Wn,
+                                                       float4* kstate, int kfirst,
+                                                       float* blocks) {
+  using S = Shape<BW, BH>;
+  __shared__ c32 lds[S::LDS_C];
+  const int lane = threadIdx.x, g = lane / S::LPB, j = lane - g * S::LPB;
+  const int b = blockIdx.x * S::G + g;
+  const bool live = b < P.nb;
+  c32* Yg = lds + g * S::FBW * S::YP;
+  if (live) filter_block<BW, BH>(P, Wn, b, j, Yg, kstate, kfirst != 0);
+  __syncthreads();
+  repack_block<BW, BH>(live, j, Yg);
+  float xr[BW], xi[BW];
+  inverse_block<BW, BH>(live, j, Yg, xr, xi);
+  if (live && j < S::NROW) {
+    int x0, y0, par;
+    block_origin(P, b, x0, y0, par);
+    const float* win = par ? P.winSynB : P.winSynA;
+    float* dst = blocks + (size_t)b * BW * BH + (2 * j) * BW;
+#pragma unroll
+    for (int k = 0; k < BW; k++) {
+      dst[k] = xr[k] * win[2 * j * BW + k];
+      dst[BW + k] = xi[k] * win[(2 * j + 1) * BW + k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- synthesis
+
+__device__ __forceinline__ int floordiv(int a, int b) {
+  int q = a / b;
+  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
+// Float -> sample (src/cpu/reference.cpp:107-138; src/cl/kernels.h:436-447).
+__device__ __forceinline__ void store_sample(const PlaneArgs& P, void* out, size_t i, float v) {
+#pragma clang fp contract(off)
+  if (P.pix == 8) {
+    v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    static_cast<uint8_t*>(out)[i] = (uint8_t)floorf(v * 255.0f + 0.5f);
+  } else if (P.pix == 16) {
+    v = v * P.invNorm;
+    v = v < P.maxPix ? v : P.maxPix;
+    v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    static_cast<uint16_t*>(out)[i] = (uint16_t)floorf(v * 65535.0f + 0.5f);
+  } else {
+    static_cast<float*>(out)[i] = v;
+  }
+}
+
+// Overlap-add of the windowed blocks covering each pixel, in block-index
+// order (src/cpu/reference.cpp:415-436), plus chroma centring, then pack.
+__global__ __launch_bounds__(256) void k_synthesize(PlaneArgs P, const float* blocks,
+                                                    void* out) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= P.W || y >= P.H) return;
+  const int bw = P.bw, bh = P.bh;
+  float sum = 0.0f;
+  if (P.mode == 1) {
+    const int xs = bw - P.ow, ys = bh - P.oh;
+    int n0 = floordiv(x + P.ow - bw + 1, xs), n1 = floordiv(x + P.ow, xs);
+    int m0 = floordiv(y + P.oh - bh + 1, ys), m1 = floordiv(y + P.oh, ys);
+    if (n0 < 0) n0 = 0;
+    if (m0 < 0) m0 = 0;
+    if (n1 >= P.ncols) n1 = P.ncols - 1;
+    if (m1 >= P.nrows) m1 = P.nrows - 1;
+    for (int m = m0; m <= m1; m++) {
+      const int ly = y + P.oh - m * ys;
+      for (int n = n0; n <= n1; n++) {
+        const int lx = x + P.ow - n * xs;
+        if (lx < 0 || lx >= bw || ly < 0 || ly >= bh) continue;
+        sum += blocks[((size_t)m * P.ncols + n) * bw * bh + ly * bw + lx];
+      }
+    }
+  } else {
+    for (int grid = 0; grid < 2; grid++) {
+      const int sx = x + (grid ? bw / 2 : 0), sy = y + (grid ? bh / 2 : 0);
+      const int i = sx / bw, j = sy / bh;
+      if (i < P.nx && j < P.ny) {
+        const size_t b = (size_t)grid * P.nx * P.ny + (size_t)j * P.nx + i;
+        sum += blocks[b * bw * bh + (sy - j * bh) * bw + (sx - i * bw)];
+      }
+    }
+  }
+  if (P.chromaNorm != 0.0f) sum += P.chromaNorm;
+  store_sample(P, out, (size_t)y * P.W + x, sum);
+}
+
+} // namespace fk

@@ -1,0 +1,142 @@
+/*
+ * fft3d_hip.h - C ABI of the MI355X (gfx950) backend of the FFT3D denoise path.
+ *
+ * This is the operator boundary that replaces the reference's OpenCL backend
+ * class GpuRef (src/cl/gpu.h:16-88) behind the VapourSynth filter. Every entry
+ * point names the reference interface it stands in for. Conventions:
+ *   - plain C types only; no C++ exceptions cross the boundary;
+ *   - functions return 0 (FFT3D_OK) or a negative FFT3D_E* code; a plane
+ *     object keeps the reason as a sticky string (fft3d_plane_error), the
+ *     reference's bool + err_ convention;
+ *   - a plane object is used by one thread at a time (the reference holds its
+ *     instance mutex around every call, src/fft3dcl.cpp:366);
+ *   - `stream` is a hipStream_t (NULL = the plane's own stream). All device
+ *     work of a call is ordered on that stream; calls with host buffers
+ *     return after the host side is complete.
+ */
+#ifndef FFT3D_HIP_H
+#define FFT3D_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFT3D_ABI_VERSION 1
+
+enum {
+  FFT3D_OK = 0,
+  FFT3D_EINVAL = -1,   /* bad argument or parameter (message says which) */
+  FFT3D_EDEVICE = -2,  /* no usable HIP device */
+  FFT3D_ENOMEM = -3,   /* device allocation failed */
+  FFT3D_ELAUNCH = -4,  /* kernel launch or HIP runtime failure */
+  FFT3D_EMISSING = -5, /* a window frame has no spectrum */
+  FFT3D_EUNSUPPORTED = -6 /* parameter combination the GPU path does not run */
+};
+
+/* Script parameters of FFT3DCL(clip, ...), user units
+ * (ScriptParams, src/params.h:8-27; defaults src/fft3dcl.cpp:171-195). */
+typedef struct fft3d_params {
+  float sigma, beta;
+  int bw, bh, bt;
+  float sharpen;
+  int plane, mode, bordersize, precision;
+  float degrid, scutoff, svr, smin, smax, kratio;
+  int ow, oh, wintype, interlaced;
+  float sigma2, sigma3, sigma4;
+  int oldfft;
+} fft3d_params;
+
+/* One plane of a clip (PlaneGeom, src/cpu/reference.h:17-22). */
+typedef struct fft3d_plane_desc {
+  int width, height;
+  int bits;        /* 8..16 for integer samples, 32 for float */
+  int sample_type; /* 0 integer, 1 float */
+  int chroma;      /* YUV chroma plane (mode-1 centring) */
+} fft3d_plane_desc;
+
+typedef struct fft3d_plane fft3d_plane;
+
+/* Derived geometry and footprint of a plane object. */
+typedef struct fft3d_plane_info {
+  int nb;            /* blocks per frame */
+  int bw, bh, ow, oh, mode, fbw;
+  int bm;            /* temporal mode -1..4 (the script's bt) */
+  int bt;            /* frames in the window (max(bm,1)) */
+  int current;       /* index of frame n inside the window */
+  int ncols, nrows;  /* block grid */
+  int blocks_per_wg; /* blocks one 64-lane workgroup transforms */
+  size_t spectrum_bytes; /* one frame's spectrum in the ring */
+  size_t device_bytes;   /* all device memory the plane holds */
+  int cached_frames;     /* spectra currently in the ring */
+} fft3d_plane_info;
+
+/* Fills the reference's defaults for a clip with no keys set (degrid 1, ow/oh
+ * 16 for bw/bh 32, sigma2/3/4 = sigma). */
+void fft3d_params_default(fft3d_params* p);
+
+/* Selects and probes HIP device `device` (GpuContext::init,
+ * src/cl/cl.cpp:138-181). Returns FFT3D_EDEVICE with a reason when there is
+ * no usable gfx950 device. */
+int fft3d_dev_init(int device, char* err, size_t errlen);
+
+/* Validates parameters for this plane (validateParams, src/params.h:76-148)
+ * and builds the plane object: tables, block grid, Kalman state, staging
+ * (GpuRef::GpuRef, src/cl/gpu.cpp:59-245). On failure *out is NULL and err
+ * holds the reference's message ("FFT3DCL: ..."). */
+int fft3d_plane_create(const fft3d_params* params, const fft3d_plane_desc* desc,
+                       fft3d_plane** out, char* err, size_t errlen);
+
+/* Releases everything the plane holds (GpuRef::~GpuRef, src/cl/gpu.cpp:247-270). */
+void fft3d_plane_destroy(fft3d_plane* p);
+
+/* Last error of the plane (GpuRef::error, src/cl/gpu.h:23). */
+const char* fft3d_plane_error(const fft3d_plane* p);
+
+/* 1 when frame fn's spectrum is in the ring (GpuRef::has, src/cl/gpu.h:35). */
+int fft3d_plane_has(const fft3d_plane* p, int fn);
+
+/* Analysis of frame fn: window + 2-D R2C FFT of every block into the ring
+ * (GpuRef::forwardRaw / forward, src/cl/gpu.cpp:325-372). src holds the plane
+ * as packed samples, width*height*bytes_per_sample bytes (u8, u16 or f32);
+ * src_on_device = 1 reads it in place from device memory. */
+int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes,
+                        int src_on_device, void* stream);
+
+/* Filter + inverse + overlap-add synthesis of the output frame for the
+ * temporal window `window` (clamped frame numbers in temporal order: bt
+ * entries for bt >= 2, else one), written to dst as packed samples
+ * (GpuRef::processRaw / process, src/cl/gpu.cpp:474-505). */
+int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst,
+                        size_t bytes, int dst_on_device, void* stream);
+
+/* Drops ring entries outside [lo, hi]; a negative bound is open
+ * (GpuRef::pruneCache, src/cl/gpu.cpp:507-517). Retired spectra go to a
+ * bounded reuse pool. */
+void fft3d_plane_prune(fft3d_plane* p, int lo, int hi);
+
+/* Drops every ring entry (the intent of pruneCache(-1,-2) at
+ * src/fft3dcl.cpp:465, which in the reference keeps everything). */
+void fft3d_plane_clear(fft3d_plane* p);
+
+/* Process-wide live plane objects and device bytes
+ * (GpuRef::liveInstances / liveBytes, src/cl/gpu.h:27-28). */
+int fft3d_live_instances(void);
+size_t fft3d_live_bytes(void);
+
+/* Introspection for tests and tools. */
+int fft3d_plane_get_info(const fft3d_plane* p, fft3d_plane_info* info);
+/* Host copy of a constant table: 0 analysis window, 1/2 synthesis windows
+ * (even/odd block columns), 3 sigma pattern, 4 sharpen gains, 5 degrid
+ * sample (re,im interleaved). Returns the float count (0 when off). */
+int fft3d_plane_get_table(const fft3d_plane* p, int which, float* dst);
+/* Frame fn's ring spectrum unpacked to the reference layout: nb blocks x bh
+ * rows x (bw/2+1) columns of (re, im), i.e. PlaneRef::Spectra
+ * (src/cpu/reference.h:30-33) interleaved. */
+int fft3d_plane_read_spectrum(fft3d_plane* p, int fn, float* dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+// Thread-per-lane CPU emulation of the HIP subset used by fft3dcl_amd/csrc.
+//
+// Test infrastructure: lets the kernels and the C-ABI host run on a machine
+// with no GPU, so index arithmetic, LDS layouts and barriers are checked
+// (under AddressSanitizer in tests/test_emu_kernels.py) before a kernel ever
+// runs on the device. One OS thread per lane; workgroups run one after the
+// other so a kernel's __shared__ arrays (function-local statics here) are
+// private to the running workgroup. "Device" memory is host memory.
+#pragma once
+
+#include <barrier>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct dim3 {
+  unsigned x = 1, y = 1, z = 1;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct float4 {
+  float x, y, z, w;
+};
+inline float4 make_float4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
+
+inline thread_local dim3 threadIdx, blockIdx;
+inline dim3 blockDim, gridDim;
+inline std::barrier<>* emu_barrier_ = nullptr;
+
+inline void __syncthreads() { emu_barrier_->arrive_and_wait(); }
+inline float emu_sqrtf(float x) { return std::sqrt(x); }
+using std::copysign;
+using std::floor;
+inline float copysignf(float a, float b) { return std::copysign(a, b); }
+inline float floorf(float a) { return std::floor(a); }
+inline float fmaxf(float a, float b) { return std::fmax(a, b); }
+
+// Runs fn() once per lane of every workgroup of the grid.
+inline void emu_launch(dim3 grid, dim3 block, const std::function<void()>& fn) {
+  const unsigned nt = block.x * block.y * block.z;
+  blockDim = block;
+  gridDim = grid;
+  std::barrier<> bar(nt);
+  emu_barrier_ = &bar;
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (unsigned t = 0; t < nt; t++) {
+    th.emplace_back([&, t] {
+      threadIdx = dim3(t % block.x, (t / block.x) % block.y, t / (block.x * block.y));
+      for (unsigned bz = 0; bz < grid.z; bz++)
+        for (unsigned by = 0; by < grid.y; by++)
+          for (unsigned bx = 0; bx < grid.x; bx++) {
+            blockIdx = dim3(bx, by, bz);
+            fn();
+            bar.arrive_and_wait();  // the next workgroup reuses the shared arrays
+          }
+    });
+  }
+  for (auto& x : th) x.join();
+  emu_barrier_ = nullptr;
+}
+
+// ---- runtime API subset (synchronous, host memory) ----
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 };
+typedef void* hipStream_t;
+typedef void* hipEvent_t;
+enum hipMemcpyKind {
+  hipMemcpyHostToHost = 0,
+  hipMemcpyHostToDevice = 1,
+  hipMemcpyDeviceToHost = 2,
+  hipMemcpyDeviceToDevice = 3,
+  hipMemcpyDefault = 4
+};
+#define hipStreamNonBlocking 1
+struct hipDeviceProp_t {
+  char name[256];
+  size_t sharedMemPerBlock;
+  size_t totalGlobalMem;
+  int multiProcessorCount;
+  char gcnArchName[256];
+};
+
+inline hipError_t emu_fail_alloc_ = 0;  // tests set it to force allocation failures
+inline hipError_t hipMalloc(void** p, size_t n) {
+  if (emu_fail_alloc_) return hipErrorOutOfMemory;
+  *p = std::calloc(1, n ? n : 1);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipFree(void* p) {
+  std::free(p);
+  return hipSuccess;
+}
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
+  std::memcpy(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) {
+  return hipMemcpy(d, s, n, k);
+}
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+  *s = reinterpret_cast<hipStream_t>(0x1);
+  return hipSuccess;
+}
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) {
+  *d = 0;
+  return hipSuccess;
+}
+inline hipError_t hipGetDeviceCount(int* n) {
+  *n = 1;
+  return hipSuccess;
+}
+inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
+  std::memset(p, 0, sizeof(*p));
+  std::strcpy(p->name, "cpu-emulation");
+  std::strcpy(p->gcnArchName, "emu");
+  p->sharedMemPerBlock = 160 * 1024;
+  p->totalGlobalMem = (size_t)1 << 34;
+  p->multiProcessorCount = 1;
+  return hipSuccess;
+}
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipPeekAtLastError() { return hipSuccess; }
+inline const char* hipGetErrorString(hipError_t e) {
+  return e == hipSuccess ? "no error" : (e == hipErrorOutOfMemory ? "out of memory" : "error");
+}
+inline hipError_t hipEventCreate(hipEvent_t* e) {
+  *e = nullptr;
+  return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
+  *ms = 0.0f;
+  return hipSuccess;
+}
+
+#define FK_LAUNCH(fn, grid, block, stream, ...) \
+  emu_launch((grid), (block), [&]() { fn(__VA_ARGS__); })
