@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Throughput of the FFT3D denoise hot path on one MI355X.
+
+Workload: 4K 10-bit luma (3840x2160), sigma=3, bt=4, bw=bh=32, ow=oh=16,
+sharpen=0.4, degrid=1 (mode 1 default), FP32 spectra ring: the configuration
+the 4K10 frames/s metric is quoted on. A step is one output frame of the
+sequential filter (analysis of the newly needed frame into the ring, temporal
+filter over the 4-frame window, inverse FFT, overlap-add, 10-bit pack), with
+the input planes already resident in HBM and the output written to HBM.
+
+Prints one JSON line. Per-kernel device times come from HIP events recorded on
+the launch stream around every kernel during the timed region; `roofline`
+prices the dominant kernel's algorithmic bytes per launch (per-unit figures
+in DESIGN.md) against the 8 TB/s HBM peak. `cpu_baseline` times the
+reference's CPU path (its own sources built into oracle/_ref when present,
+else the oracle restatement) single-threaded on a bounded 3-frame sample of
+the same clip.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "4K10 frames/s (1 GPU), % HBM roofline; p50/p99 frame latency at 32 streams"
+W, H, BITS = 3840, 2160, 10
+PARAMS = dict(sigma=3.0, bt=4, bw=32, bh=32, ow=16, oh=16, sharpen=0.4)
+HBM_PEAK_GBS = 8000.0
+SEED = 1002
+NUNIQ = 16  # distinct synthetic frames, cycled through the clip
+
+
+def kernel_bytes(info, bps_in, bps_out):
+    """Algorithmic HBM bytes per launch of each kernel. A spectrum is
+    bins * 8 B with bins = nb * (bw/2+1) * bh (complex FP32)."""
+    bins = info.nb * info.fbw * info.bh
+    spec = bins * 8
+    return {
+        "analysis": W * H * bps_in + spec,          # read the plane, write one spectrum
+        "filter_inverse": spec * info.bt,           # read the bt-frame window
+        "synthesize": W * H * bps_out,              # write the packed plane
+        "step": W * H * (bps_in + bps_out) + spec * info.bt,
+    }
+
+
+def cpu_baseline(frames):
+    """Times the reference CPU path on output frames 0..2 (cold window: four
+    analyses + three filter/synthesis passes), one thread."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import Oracle, Ref, script
+    ref = Ref.load()
+    kind = "reference" if ref is not None else "port"
+    lib = ref if ref is not None else Oracle()
+    sample = frames[:5]
+    t0 = time.perf_counter()
+    lib.run_clip(script(**PARAMS), [sample], BITS, 0, order=[0, 1, 2])
+    dt = time.perf_counter() - t0
+    return {"value": round(3 / dt, 4), "unit": "frames/s", "cores": 1, "kind": kind,
+            "sample": "output frames 0-2 of the same 4K 10-bit clip from a cold window "
+                      f"(4 analyses + 3 filter/synthesis), {dt:.1f} s single-threaded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.stderr.write("bench.py measures one GPU only: this build has no multi-GPU path "
+                         "(streams shard across GPUs without communication; see DESIGN.md)\n")
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from fft3dcl_amd import capi
+    from fft3dcl_amd.filter import GRAY, Instance
+    from fft3dcl_amd.synth import make_plane_sequence
+
+    capi.dev_init(0)
+    frames = make_plane_sequence(SEED, W, H, NUNIQ, "u10")
+    dev = torch.from_numpy(frames).to("cuda")
+    out = torch.empty((H, W), dtype=torch.int16, device="cuda")
+    nframes = max(120, args.warmup + args.steps + 8)
+    inst = Instance(GRAY, W, H, BITS, 0, nframes, **PARAMS)
+    plane = inst.objs[0]
+    info = plane.info
+    stream = torch.cuda.Stream()
+    handle = stream.cuda_stream
+    srcs = [lambda fn: dev[fn % NUNIQ]]
+
+    def step(n):
+        inst.get_frame(n, srcs, [out], on_device=True, stream=handle)
+
+    torch.cuda.synchronize()
+    for n in range(args.warmup):
+        step(n)
+    torch.cuda.synchronize()
+    plane.set_profiling(True)
+    plane.profile()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for n in range(args.warmup, args.warmup + args.steps):
+        step(n)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_total = ev0.elapsed_time(ev1)
+    prof = plane.profile()
+    plane.set_profiling(False)
+    inst.close()
+
+    ms_step = ms_total / args.steps
+    fps = 1000.0 / ms_step
+    kb = kernel_bytes(info, 2, 2)
+    kernels = {}
+    for name, (launches, tot) in prof.items():
+        avg = tot / launches
+        gbs = kb[name] / (avg * 1e-3) / 1e9
+        kernels[name] = {"launches": launches, "avg_ms": round(avg, 4),
+                         "algorithmic_bytes": kb[name], "GBps": round(gbs, 1)}
+    dom = max(prof, key=lambda k: prof[k][1])
+    traffic = None
+    tf = ROOT / "profiles" / "traffic.json"
+    if tf.exists():
+        t = json.loads(tf.read_text()).get(dom)
+        traffic = t.get("bytes_per_launch") if isinstance(t, dict) else None
+    ach = kernels[dom]["GBps"]
+    step_gbs = kb["step"] / (ms_step * 1e-3) / 1e9
+    line = {
+        "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "4K 10-bit luma 3840x2160, sigma=3 bt=4 bw=bh=32 ow=oh=16 "
+                               "sharpen=0.4 degrid=1, FP32 spectra ring",
+                   "frames": nframes, "distinct_frames": NUNIQ, "seed": SEED,
+                   "mpix_per_s": round(fps * W * H / 1e6, 1),
+                   "step_algorithmic_bytes": kb["step"],
+                   "step_GBps": round(step_gbs, 1),
+                   "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+                   "kernels": kernels},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": traffic},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(frames)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -48,7 +48,12 @@ class PlaneInfo(C.Structure):
                 ("cached_frames", C.c_int)]
 
 
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_int), ("total_ms", C.c_double)]
+
+
 EXPORTS = [
+    "fft3d_plane_set_profiling", "fft3d_plane_get_profile",
     "fft3d_params_default", "fft3d_dev_init", "fft3d_plane_create", "fft3d_plane_destroy",
     "fft3d_plane_error", "fft3d_plane_has", "fft3d_plane_forward", "fft3d_plane_process",
     "fft3d_plane_prune", "fft3d_plane_clear", "fft3d_live_instances", "fft3d_live_bytes",
@@ -99,6 +104,8 @@ def load(path: str | Path | None = None) -> C.CDLL:
     L.fft3d_plane_get_info.argtypes = [vp, C.POINTER(PlaneInfo)]
     L.fft3d_plane_get_table.argtypes = [vp, i, vp]
     L.fft3d_plane_read_spectrum.argtypes = [vp, i, vp]
+    L.fft3d_plane_set_profiling.argtypes = [vp, i]
+    L.fft3d_plane_get_profile.argtypes = [vp, C.POINTER(KernelTime), i]
     _LIBS[path] = L
     return L
 
@@ -203,6 +210,17 @@ class Plane:
         a = np.zeros(n, np.float32)
         self.lib.fft3d_plane_get_table(self.h, which, a.ctypes.data)
         return a
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.fft3d_plane_set_profiling(self.h, int(on)))
+
+    def profile(self) -> dict:
+        """{kernel: (launches, total_ms)} since the last call (waits for the GPU)."""
+        buf = (KernelTime * 8)()
+        n = self.lib.fft3d_plane_get_profile(self.h, buf, 8)
+        if n < 0:
+            self._check(n)
+        return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms) for i in range(n)}
 
     def read_spectrum(self, fn: int) -> np.ndarray:
         i = self.info

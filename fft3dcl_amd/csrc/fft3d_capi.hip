@@ -94,7 +94,50 @@ struct fft3d_plane {
   std::map<int, fk::c32*> ring;  // frame number -> spectrum
   std::vector<fk::c32*> pool;    // retired spectra kept for reuse
 
+  // profiling: events around each launch, resolved in get_profile
+  enum { K_ANALYSIS, K_FILTER, K_SYNTH, K_COUNT };
+  struct Rec {
+    int kind;
+    hipEvent_t a, b;
+  };
+  bool prof = false;
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> evpool;
+  hipEvent_t ev() {
+    hipEvent_t e = nullptr;
+    if (!evpool.empty()) {
+      e = evpool.back();
+      evpool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      (void)hipGetLastError();
+      e = nullptr;
+    }
+    return e;
+  }
+  // brackets a launch when profiling: returns the start event (or null)
+  hipEvent_t prof_begin(hipStream_t s) {
+    if (!prof) return nullptr;
+    hipEvent_t a = ev();
+    if (a) (void)hipEventRecord(a, s);
+    return a;
+  }
+  void prof_end(hipEvent_t a, int kind, hipStream_t s) {
+    if (!a) return;
+    hipEvent_t b = ev();
+    if (!b) {
+      evpool.push_back(a);
+      return;
+    }
+    (void)hipEventRecord(b, s);
+    recs.push_back(Rec{kind, a, b});
+  }
+
   ~fft3d_plane() {
+    for (auto& r : recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : evpool) (void)hipEventDestroy(e);
     for (auto& kv : ring) (void)hipFree(kv.second);
     for (auto* m : pool) (void)hipFree(m);
     (void)hipFree(dTables);
@@ -362,7 +405,9 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
   }
   const int grid = (p->nb + p->ks.G - 1) / p->ks.G;
   AnalysisFn k = p->ks.analysis;
+  hipEvent_t t0 = p->prof_begin(s);
   FK_LAUNCH(k, dim3(grid), dim3(64), s, p->args, plane, spec);
+  p->prof_end(t0, fft3d_plane::K_ANALYSIS, s);
   if (!p->launched("fft2d forward")) {
     p->ring.erase(fn);
     p->retire(spec);
@@ -391,13 +436,17 @@ int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst, 
   const int grid = (p->nb + p->ks.G - 1) / p->ks.G;
   FilterFn kf = p->ks.filter_inverse;
   const int kfirst = (p->p.bm == 0 && p->kalmanFirst) ? 1 : 0;
+  hipEvent_t t0 = p->prof_begin(s);
   FK_LAUNCH(kf, dim3(grid), dim3(64), s, p->args, w, p->dKalman, kfirst, p->dBlocks);
+  p->prof_end(t0, fft3d_plane::K_FILTER, s);
   if (!p->launched("filter/inverse")) return FFT3D_ELAUNCH;
   if (p->p.bm == 0) p->kalmanFirst = false;
   void* outp = dst_on_device ? dst : p->dStage;
   SynthFn ksy = &fk::k_synthesize;
+  t0 = p->prof_begin(s);
   FK_LAUNCH(ksy, dim3((p->d.width + 15) / 16, (p->d.height + 15) / 16), dim3(256), s, p->args,
             p->dBlocks, outp);
+  p->prof_end(t0, fft3d_plane::K_SYNTH, s);
   if (!p->launched("synthesize")) return FFT3D_ELAUNCH;
   if (!dst_on_device) {
     if (hipMemcpyAsync(dst, p->dStage, need, hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -426,6 +475,42 @@ void fft3d_plane_clear(fft3d_plane* p) {
   if (!p) return;
   for (auto& kv : p->ring) p->retire(kv.second);
   p->ring.clear();
+}
+
+int fft3d_plane_set_profiling(fft3d_plane* p, int on) {
+  if (!p) return FFT3D_EINVAL;
+  p->prof = on != 0;
+  return FFT3D_OK;
+}
+
+int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max) {
+  if (!p) return FFT3D_EINVAL;
+  static const char* names[fft3d_plane::K_COUNT] = {"analysis", "filter_inverse", "synthesize"};
+  double ms[fft3d_plane::K_COUNT] = {0, 0, 0};
+  int n[fft3d_plane::K_COUNT] = {0, 0, 0};
+  for (auto& r : p->recs) {
+    float t = 0.0f;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+      ms[r.kind] += t;
+      n[r.kind]++;
+    } else {
+      (void)hipGetLastError();
+    }
+    p->evpool.push_back(r.a);
+    p->evpool.push_back(r.b);
+  }
+  p->recs.clear();
+  int k = 0;
+  for (int i = 0; i < fft3d_plane::K_COUNT && k < max; i++) {
+    if (!n[i]) continue;
+    if (out) {
+      std::snprintf(out[k].name, sizeof(out[k].name), "%s", names[i]);
+      out[k].launches = n[i];
+      out[k].total_ms = ms[i];
+    }
+    k++;
+  }
+  return k;
 }
 
 int fft3d_live_instances(void) { return g_live.load(); }

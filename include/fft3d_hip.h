@@ -125,6 +125,19 @@ void fft3d_plane_clear(fft3d_plane* p);
 int fft3d_live_instances(void);
 size_t fft3d_live_bytes(void);
 
+/* Per-kernel device time, measured with HIP events recorded on the launch
+ * stream around every kernel while profiling is on (the reference's
+ * FFT3DCL_TIME stage table, src/timing.h:12-65, without draining the queue). */
+typedef struct fft3d_kernel_time {
+  char name[32];
+  int launches;
+  double total_ms;
+} fft3d_kernel_time;
+int fft3d_plane_set_profiling(fft3d_plane* p, int on);
+/* Waits for the recorded events, then returns per-kernel totals since the
+ * last call (up to max entries) and resets them. */
+int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max);
+
 /* Introspection for tests and tools. */
 int fft3d_plane_get_info(const fft3d_plane* p, fft3d_plane_info* info);
 /* Host copy of a constant table: 0 analysis window, 1/2 synthesis windows

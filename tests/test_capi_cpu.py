@@ -1,0 +1,178 @@
+"""CPU tests of the C ABI (include/fft3d_hip.h).
+
+* The gfx950 library loads and exports every entry point the header declares
+  (no compute call is made without a GPU).
+* The same C ABI and kernels, compiled against the CPU emulation of HIP
+  (tests/emu), agree with the oracle: this checks the kernels' index
+  arithmetic, LDS layouts, packing and the host scheduling on any machine.
+* Host logic: parameter validation messages, plane routing, tables.
+"""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fft3dcl_amd import capi
+from fft3dcl_amd.filter import FFT3DCL, route_planes
+from fft3dcl_amd.synth import make_clip
+from helpers import lsb_diff, max_abs, mt19937_planes
+from oracle_lib import Plane as OPlane, script, kind_info
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "fft3d_hip.h"
+
+
+def declared_symbols():
+    txt = HEADER.read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fft3d_\w+)\s*\(", txt)))
+
+
+def test_header_declares_the_binding_exports():
+    assert declared_symbols() == sorted(capi.EXPORTS)
+
+
+def test_hip_library_exports_every_declared_symbol(hip_lib):
+    lib = C.CDLL(hip_lib)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_emu_library_exports_every_declared_symbol(emu_lib):
+    lib = C.CDLL(emu_lib)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_params_default_matches_reference_defaults(emu_lib):
+    lib = capi.load(emu_lib)
+    p = capi.Params()
+    lib.fft3d_params_default(C.byref(p))
+    want = capi.make_params()
+    for n in capi.PARAM_NAMES:
+        assert getattr(p, n) == getattr(want, n), n
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(bt=7), "FFT3DCL: valid modes for bt are -1,0,1,2,3,4"),
+    (dict(mode=2), "FFT3DCL: mode 2 is not implemented yet in this port (use mode 0 or 1)"),
+    (dict(interlaced=1), "FFT3DCL: interlaced is not implemented yet in this port"),
+    (dict(bw=128), "FFT3DCL: GPU setup failed: block too large for the GPU path (max 64x64)"),
+])
+def test_create_errors(emu_lib, kw, msg):
+    with pytest.raises(capi.FFT3DError) as e:
+        capi.Plane(capi.make_params(**kw), 64, 48, 8, lib=emu_lib)
+    assert str(e.value) == msg
+
+
+def test_format_errors(emu_lib):
+    with pytest.raises(capi.FFT3DError, match="integer bit depths 8-16 only"):
+        capi.Plane(capi.make_params(), 64, 48, 7, lib=emu_lib)
+    with pytest.raises(capi.FFT3DError, match="only 32-bit float clips"):
+        capi.Plane(capi.make_params(), 64, 48, 16, sample_type=1, lib=emu_lib)
+
+
+def test_plane_routing():
+    assert route_planes(1, 0) == [0] and route_planes(1, 4) == [0]
+    assert route_planes(3, 1) == [1, 2] and route_planes(3, 3) == [1, 2]
+    assert route_planes(3, 4) == [0, 1, 2] and route_planes(2, 0) == [0, 1, 2]
+    with pytest.raises(ValueError, match="cannot set chroma processing for a greyscale clip"):
+        route_planes(1, 1)
+    with pytest.raises(ValueError, match=r"plane must be 0 \(luma\), 1 \(chroma\) or 4 \(all\)"):
+        route_planes(3, 5)
+
+
+def test_device0_is_rejected(emu_lib):
+    planes = make_clip(1, 64, 48, 2, "u8")
+    with pytest.raises(ValueError, match="device=0"):
+        FFT3DCL(planes, bits=8, device=0, lib=emu_lib)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(sharpen=0.5, sigma2=3.0, sigma3=1.0, sigma4=2.0),
+                                dict(mode=0, bw=16, bh=32), dict(wintype=1, ow=8, oh=12)])
+def test_product_tables_match_oracle(oracle, emu_lib, kw):
+    sp = script(**kw)
+    pl = capi.Plane(capi.make_params(**kw), 120, 90, 8, lib=emu_lib)
+    op = OPlane(oracle, sp, 120, 90, 8)
+    op.build_degrid()
+    t = op.tables()
+    assert np.array_equal(pl.table(0), t["winAna"].ravel())
+    if sp.mode == 1:
+        assert np.array_equal(pl.table(1), t["winSynA"].ravel())
+        assert np.array_equal(pl.table(2), t["winSynB"].ravel())
+    if t["pattern"] is not None:
+        assert np.array_equal(pl.table(3), t["pattern"].ravel())
+    if t["sharpen"] is not None:
+        assert np.array_equal(pl.table(4), t["sharpen"].ravel())
+    if sp.degrid != 0.0:
+        g = pl.table(5).view(np.complex64)
+        assert np.abs(g - t["degrid"].ravel()).max() < 1e-6
+
+
+@pytest.mark.parametrize("bw,bh", [(4, 4), (8, 16), (32, 32), (64, 64), (32, 8)])
+def test_emu_spectrum_matches_oracle(oracle, emu_lib, bw, bh):
+    planes = make_clip(2, 90, 70, 1, "f32")
+    pl = capi.Plane(capi.make_params(bw=bw, bh=bh), 90, 70, 32, sample_type=1, lib=emu_lib)
+    pl.forward(0, np.ascontiguousarray(planes[0][0]))
+    got = pl.read_spectrum(0)
+    op = OPlane(oracle, script(bw=bw, bh=bh), 90, 70, 32, sample_type=1)
+    want = op.spectra(planes[0][0])
+    assert np.abs(got - want).max() < 2e-5 * np.abs(want).max()
+
+
+EMU_CASES = [
+    ("u8", dict(bt=3)), ("u8", dict(bt=-1, sharpen=0.8)), ("u10", dict(bt=4, sharpen=0.4)),
+    ("u8", dict(bt=2)), ("u8", dict(bt=1, sigma2=3.0, sigma3=1.5, sigma4=1.0)),
+    ("f32", dict(bt=0, kratio=1.5)), ("u8", dict(bt=3, mode=0)),
+    ("u10", dict(bt=3, bw=16, bh=16)), ("u16", dict(bt=3, bw=64, bh=32, ow=24, oh=8)),
+]
+
+
+@pytest.mark.parametrize("kind,kw", EMU_CASES)
+def test_emu_clip_matches_oracle(oracle, emu_lib, kind, kw):
+    bits, st = kind_info(kind)
+    planes = make_clip(3, 80, 56, 5, kind)
+    order = [0, 1, 2, 4, 3]
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st, order=order)
+    got = FFT3DCL(planes, bits=bits, sample_type=st, order=order, lib=emu_lib, **kw)
+    if kind == "f32":
+        assert max_abs(got[0], want[0]) < 1e-5
+    else:
+        assert lsb_diff(got[0], want[0]) <= 1
+
+
+def test_emu_yuv_plane4_matches_oracle(oracle, emu_lib):
+    planes = make_clip(4, 96, 64, 4, "u8", color="yuv")
+    want, _ = oracle.run_clip(script(plane=4, bt=3), planes, 8, 0, color_family=3)
+    got = FFT3DCL(planes, color_family=3, bits=8, plane=4, bt=3, lib=emu_lib)
+    for g, w in zip(got, want):
+        assert lsb_diff(g, w) <= 1
+
+
+def test_emu_gpu_test_inputs(oracle, emu_lib):
+    # tools/gpu_test.cpp: 256x192 mt19937(42) planes, float domain, PASS < 2e-4
+    for bt in (1, 2, 3, 4, 0):
+        pl = mt19937_planes(42, 256, 192, max(bt, 1))
+        planes = [np.stack(pl)]
+        _, want = oracle.run_clip(script(bt=bt), planes, 32, 1, want_float=True)
+        got = FFT3DCL(planes, bits=32, sample_type=1, bt=bt, lib=emu_lib)
+        n = 0 if bt < 2 else (2 if bt == 4 else 1)
+        assert max_abs(got[0][n], want[0][n]) < 2e-4
+
+
+def test_emu_ring_prune_and_counts(emu_lib):
+    planes = make_clip(5, 64, 48, 8, "u8")
+    inst_before = capi.live_instances(emu_lib)
+    from fft3dcl_amd.filter import Instance
+    inst = Instance(1, 64, 48, 8, 0, 8, bt=4, lib=emu_lib)
+    srcs = [lambda fn: np.ascontiguousarray(planes[0][fn])]
+    out = np.zeros((48, 64), np.uint8)
+    for n in range(8):
+        inst.get_frame(n, srcs, [out])
+        # keeps [n-current-1, n-current+bt] (src/fft3dcl.cpp:462), i.e. <= bt+2
+        assert inst.objs[0].info.cached_frames <= 6
+    assert capi.live_instances(emu_lib) == inst_before + 1
+    inst.close()
+    assert capi.live_instances(emu_lib) == inst_before

@@ -1,0 +1,160 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle.
+
+Tolerances (written here, from the parity bar of the path): +/-2 LSB at 8 bit,
++/-4 LSB at 10/16 bit, 1e-3 absolute for float clips (values in [0,1] scale),
+and 2e-4 in the normalised float domain for the reference's own GPU-vs-CPU
+check (tools/gpu_test.cpp:262-272). Output must be bit-identical run to run.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fft3dcl_amd import capi
+from fft3dcl_amd.filter import FFT3DCL, Instance
+from fft3dcl_amd.synth import make_clip, make_plane_sequence
+from helpers import lsb_diff, max_abs, mt19937_planes, tolerance
+from oracle_lib import script, kind_info
+
+pytestmark = pytest.mark.gpu
+
+
+def check(kind, got, want):
+    if kind == "f32":
+        d = max_abs(got, want)
+    else:
+        d = lsb_diff(got, want)
+    assert d <= tolerance(kind), d
+    return d
+
+
+@pytest.mark.parametrize("bt", [1, 2, 3, 4, 0])
+@pytest.mark.parametrize("variant", [dict(), dict(sharpen=0.7), dict(degrid=0.0),
+                                     dict(sigma2=3.0)])
+def test_reference_gpu_test(gpu, oracle, bt, variant):
+    """tools/gpu_test.cpp: 256x192 mt19937(42) float planes, default params with
+    bt / sharpen / degrid / sigma2 from argv; PASS if max |GPU-CPU| < 2e-4."""
+    planes = [np.stack(mt19937_planes(42, 256, 192, max(bt, 1)))]
+    _, want = oracle.run_clip(script(bt=bt, **variant), planes, 32, 1, want_float=True)
+    got = FFT3DCL(planes, bits=32, sample_type=1, bt=bt, lib=gpu, **variant)
+    n = 0 if bt < 2 else (2 if bt == 4 else 1)
+    assert max_abs(got[0][n], want[0][n]) < 2e-4
+
+
+CONFIGS = {
+    # plain method, CPU-runnable: 640x360 u8, 16 frames, sigma 2, bt 3, bw 32
+    "A_640x360_u8_bt3": ("u8", 640, 360, 16, "gray", dict(sigma=2.0, beta=1.0, bt=3),
+                         None),
+    # 1080p YUV420P10 plane=4, bt 4, sharpen (8 of its 64 frames)
+    "B_1080p10_yuv_plane4_bt4": ("u10", 1920, 1080, 6, "yuv",
+                                 dict(sigma=3.0, bt=4, sharpen=0.4, scutoff=0.3, svr=1.0,
+                                      smin=4.0, smax=20.0, plane=4), None),
+    # 4K 10-bit luma bt 4 sharpen (5 of its 120 frames)
+    "C_4k10_bt4": ("u10", 3840, 2160, 5, "gray", dict(sigma=3.0, bt=4, sharpen=0.4), None),
+    # 1440p float Kalman + pattern, 64x64 blocks, scene cut (8 frames, cut at 4)
+    "D_1440p_f32_kalman_bw64": ("f32", 2560, 1440, 8, "gray",
+                                dict(bt=0, kratio=2.0, bw=64, bh=64, sigma=2.0, sigma2=3.0,
+                                     sigma3=2.0, sigma4=1.0), 4),
+}
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_config_vs_oracle(gpu, oracle, name):
+    kind, W, H, n, color, kw, cut = CONFIGS[name]
+    bits, st = kind_info(kind)
+    planes = make_clip(1000 + len(name), W, H, n, kind, color=color, scene_cut=cut)
+    cf = 3 if color == "yuv" else 1
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st, color_family=cf)
+    got = FFT3DCL(planes, color_family=cf, bits=bits, sample_type=st, lib=gpu, **kw)
+    for g, w in zip(got, want):
+        d = check(kind, g, w)
+        print(f"{name}: max diff {d}")
+
+
+EDGE = [
+    ("u8", 64, 48, 1, dict(bt=3)),            # one-frame clip: window clamps to [0,0,0]
+    ("u8", 64, 48, 2, dict(bt=4)),            # two frames, bt 4
+    ("u8", 64, 48, 3, dict(bt=3)),
+    ("u8", 20, 12, 3, dict(bt=3)),            # plane smaller than one block
+    ("u10", 77, 45, 4, dict(bt=2, sharpen=0.5)),  # ragged size
+    ("u16", 96, 72, 4, dict(bt=3, beta=2.0)),
+    ("u8", 96, 72, 4, dict(bt=3, mode=0)),
+    ("u8", 96, 72, 4, dict(bt=3, bw=32, bh=16, ow=10, oh=4, wintype=1)),
+    ("u8", 96, 72, 4, dict(bt=1, bw=8, bh=8, wintype=2)),
+    ("u8", 96, 72, 4, dict(bt=3, bw=4, bh=4)),
+    ("u8", 160, 128, 4, dict(bt=3, bw=64, bh=16)),
+    ("f32", 96, 72, 4, dict(bt=-1, sharpen=1.0)),
+    ("u8", 96, 72, 5, dict(bt=0, sigma2=4.0, sigma3=2.0, sigma4=1.0)),
+]
+
+
+@pytest.mark.parametrize("kind,W,H,n,kw", EDGE)
+def test_edge_cases_vs_oracle(gpu, oracle, kind, W, H, n, kw):
+    bits, st = kind_info(kind)
+    planes = make_clip(7, W, H, n, kind)
+    order = list(range(n))[::-1] if n > 2 else None  # backwards seek too
+    if kw.get("bt") == 0:
+        order = None  # Kalman runs frames in order (fmFrameState)
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st, order=order)
+    got = FFT3DCL(planes, bits=bits, sample_type=st, order=order, lib=gpu, **kw)
+    check(kind, got[0], want[0])
+
+
+def test_yuv_chroma_only(gpu, oracle):
+    planes = make_clip(8, 128, 96, 4, "u8", color="yuv")
+    want, _ = oracle.run_clip(script(plane=1, bt=3), planes, 8, 0, color_family=3)
+    got = FFT3DCL(planes, color_family=3, bits=8, plane=1, bt=3, lib=gpu)
+    for g, w in zip(got, want):
+        check("u8", g, w)
+    assert np.array_equal(got[0], planes[0])  # untouched luma copied
+
+
+def test_run_to_run_bit_identical(gpu):
+    planes = make_clip(9, 640, 360, 6, "u10")
+    a = FFT3DCL(planes, bits=10, bt=4, sharpen=0.4, lib=gpu)
+    b = FFT3DCL(planes, bits=10, bt=4, sharpen=0.4, lib=gpu)
+    assert np.array_equal(a[0], b[0])
+    f = make_clip(9, 256, 192, 6, "f32")
+    a = FFT3DCL(f, bits=32, sample_type=1, bt=0, lib=gpu)
+    b = FFT3DCL(f, bits=32, sample_type=1, bt=0, lib=gpu)
+    assert np.array_equal(a[0], b[0])
+
+
+def test_device_resident_path_equals_host_path(gpu):
+    W, H, n = 1280, 720, 6
+    frames = make_plane_sequence(10, W, H, n, "u10")
+    host = FFT3DCL([frames], bits=10, bt=4, lib=gpu)[0]
+    dev_in = torch.from_numpy(frames).cuda()
+    dev_out = torch.zeros((n, H, W), dtype=torch.int16, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    inst = Instance(1, W, H, 10, 0, n, bt=4, lib=gpu)
+    srcs = [lambda fn: dev_in[fn]]
+    for k in range(n):
+        inst.get_frame(k, srcs, [dev_out[k]], on_device=True, stream=stream)
+    torch.cuda.synchronize()
+    inst.close()
+    assert np.array_equal(dev_out.cpu().numpy().view(np.uint16), host)
+
+
+def test_spectrum_vs_oracle(gpu, oracle):
+    from oracle_lib import Plane as OPlane
+    frames = make_plane_sequence(11, 333, 201, 1, "f32")
+    for bw, bh in ((32, 32), (16, 64), (64, 64), (8, 8)):
+        pl = capi.Plane(capi.make_params(bw=bw, bh=bh), 333, 201, 32, sample_type=1, lib=gpu)
+        pl.forward(0, np.ascontiguousarray(frames[0]))
+        got = pl.read_spectrum(0)
+        want = OPlane(oracle, script(bw=bw, bh=bh), 333, 201, 32, sample_type=1).spectra(frames[0])
+        assert np.abs(got - want).max() < 1e-5 * np.abs(want).max()
+        pl.close()
+
+
+def test_profiling_counts_launches(gpu):
+    frames = make_plane_sequence(12, 640, 360, 5, "u8")
+    inst = Instance(1, 640, 360, 8, 0, 5, bt=3, lib=gpu)
+    inst.objs[0].set_profiling(True)
+    out = np.zeros((360, 640), np.uint8)
+    for k in range(5):
+        inst.get_frame(k, [lambda fn: np.ascontiguousarray(frames[fn])], [out])
+    prof = inst.objs[0].profile()
+    assert prof["filter_inverse"][0] == 5 and prof["analysis"][0] == 5
+    assert all(ms > 0 for _, ms in prof.values())
+    inst.close()
