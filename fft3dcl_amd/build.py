@@ -40,8 +40,22 @@ def build_hip(out: Path = HIP_LIB) -> Path:
     return out
 
 
+VS_LIB = PKG / "libfft3dcl_vs.so"
+
+
+def build_plugin(out: Path = VS_LIB) -> Path:
+    """The VapourSynth API 4 plugin; loads libfft3d_hip.so from its own folder."""
+    src = CSRC / "fft3dcl_vs.cpp"
+    tmp = out.with_suffix(".so.tmp")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-fvisibility=hidden",
+          f"-I{INCLUDE}", f"-I{CSRC}", str(src), f"-L{PKG}", "-l:libfft3d_hip.so",
+          "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> list[Path]:
-    return [build_hip()]
+    return [build_hip(), build_plugin()]
 
 
 if __name__ == "__main__":
