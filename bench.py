@@ -45,6 +45,10 @@ def kernel_bytes(info, bps_in, bps_out):
         "analysis": W * H * bps_in + spec,          # read the plane, write one spectrum
         "filter_inverse": spec * info.bt,           # read the bt-frame window
         "synthesize": W * H * bps_out,              # write the packed plane
+        # fused filter/inverse/overlap-add: read the window, write the plane
+        # (the seam tiles' share of the plane is written by combine_seams)
+        "filter_synth_sweep": spec * info.bt + W * H * bps_out,
+        "combine_seams": 0,
         "step": W * H * (bps_in + bps_out) + spec * info.bt,
     }
 
@@ -72,6 +76,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB, --steps times (the "
+                         "reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -84,6 +91,18 @@ def main():
     from fft3dcl_amd import capi
     from fft3dcl_amd.filter import GRAY, Instance
     from fft3dcl_amd.synth import make_plane_sequence
+
+    if args.copy_calibration:
+        n = args.copy_calibration << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "launches": args.steps}), flush=True)
+        return
 
     capi.dev_init(0)
     frames = make_plane_sequence(SEED, W, H, NUNIQ, "u10")
@@ -124,9 +143,9 @@ def main():
     kernels = {}
     for name, (launches, tot) in prof.items():
         avg = tot / launches
-        gbs = kb[name] / (avg * 1e-3) / 1e9
+        gbs = kb.get(name, 0) / (avg * 1e-3) / 1e9
         kernels[name] = {"launches": launches, "avg_ms": round(avg, 4),
-                         "algorithmic_bytes": kb[name], "GBps": round(gbs, 1)}
+                         "algorithmic_bytes": kb.get(name, 0), "GBps": round(gbs, 1)}
     dom = max(prof, key=lambda k: prof[k][1])
     traffic = None
     tf = ROOT / "profiles" / "traffic.json"

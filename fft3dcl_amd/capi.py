@@ -53,7 +53,7 @@ class KernelTime(C.Structure):
 
 
 EXPORTS = [
-    "fft3d_plane_set_profiling", "fft3d_plane_get_profile",
+    "fft3d_plane_set_profiling", "fft3d_plane_get_profile", "fft3d_plane_set_option",
     "fft3d_params_default", "fft3d_dev_init", "fft3d_plane_create", "fft3d_plane_destroy",
     "fft3d_plane_error", "fft3d_plane_has", "fft3d_plane_forward", "fft3d_plane_process",
     "fft3d_plane_prune", "fft3d_plane_clear", "fft3d_live_instances", "fft3d_live_bytes",
@@ -105,6 +105,7 @@ def load(path: str | Path | None = None) -> C.CDLL:
     L.fft3d_plane_get_table.argtypes = [vp, i, vp]
     L.fft3d_plane_read_spectrum.argtypes = [vp, i, vp]
     L.fft3d_plane_set_profiling.argtypes = [vp, i]
+    L.fft3d_plane_set_option.argtypes = [vp, i, i]
     L.fft3d_plane_get_profile.argtypes = [vp, C.POINTER(KernelTime), i]
     _LIBS[path] = L
     return L
@@ -210,6 +211,11 @@ class Plane:
         a = np.zeros(n, np.float32)
         self.lib.fft3d_plane_get_table(self.h, which, a.ctypes.data)
         return a
+
+    OPT_SWEEP, OPT_BAND_ROWS = 1, 2
+
+    def set_option(self, option: int, value: int):
+        self._check(self.lib.fft3d_plane_set_option(self.h, option, int(value)))
 
     def set_profiling(self, on: bool):
         self._check(self.lib.fft3d_plane_set_profiling(self.h, int(on)))

@@ -5,7 +5,7 @@
 // by design: tables and grid are built here from the validated parameters
 // (no CPU backend object is needed); the Kalman first-frame init is folded
 // into the filter launch; spectra are stored packed (fk_kernels.h).
-#include "fk_kernels.h"
+#include "fk_sweep.h"
 #include "fft3d_params.h"
 #include "fft3d_hip.h"
 
@@ -26,19 +26,30 @@ std::atomic<size_t> g_bytes{0};
 using AnalysisFn = void (*)(fk::PlaneArgs, const void*, fk::c32*);
 using FilterFn = void (*)(fk::PlaneArgs, fk::WindowArgs, float4*, int, float*);
 using SynthFn = void (*)(fk::PlaneArgs, const float*, void*);
+using SweepFn = void (*)(fk::PlaneArgs, fk::WindowArgs, float4*, int, fk::SweepArgs);
+using CombineFn = void (*)(fk::PlaneArgs, fk::SweepArgs, int);
 
 struct KernelSet {
   AnalysisFn analysis = nullptr;
   FilterFn filter_inverse = nullptr;
+  SweepFn sweep = nullptr;
   int G = 0;
+  int sweepThreads = 0;  // 64 * waves per sweep workgroup
+  int nbw = 0;           // block columns per sweep strip
 };
 
+// Sweep workgroups hold WPW waves; 4 unless the per-wave LDS of large blocks
+// would leave room for only one workgroup per CU.
 template <int BW, int BH>
 KernelSet make_set() {
+  constexpr int WPW = (BW * BH >= 4096) ? 2 : 4;
   KernelSet k;
   k.analysis = &fk::k_analysis<BW, BH>;
   k.filter_inverse = &fk::k_filter_inverse<BW, BH>;
+  k.sweep = &fk::k_filter_synth<BW, BH, WPW>;
   k.G = fk::Shape<BW, BH>::G;
+  k.sweepThreads = 64 * WPW;
+  k.nbw = fk::SweepShape<BW, BH, WPW>::NBW;
   return k;
 }
 
@@ -87,15 +98,24 @@ struct fft3d_plane {
 
   float* dTables = nullptr;  // windows, pattern, sharpen, degrid
   void* dStage = nullptr;    // host-upload / host-readback staging (W*H*bps)
-  float* dBlocks = nullptr;  // windowed blocks awaiting overlap-add
+  float* dBlocks = nullptr;  // windowed blocks awaiting overlap-add (unfused path)
   float4* dKalman = nullptr; // bm == 0 state
   bool kalmanFirst = true;
+
+  // fused sweep path
+  bool sweepOk = false;      // geometry allows it
+  bool sweepOn = true;       // option
+  int bandRows = 0;          // option, 0 = automatic
+  float* dPart = nullptr;    // 4 partial planes
+  int2* dSeams = nullptr;    // border tiles of the current band layout
+  int nseams = 0, seamR = 0;
+  fk::SweepArgs sw{};
 
   std::map<int, fk::c32*> ring;  // frame number -> spectrum
   std::vector<fk::c32*> pool;    // retired spectra kept for reuse
 
   // profiling: events around each launch, resolved in get_profile
-  enum { K_ANALYSIS, K_FILTER, K_SYNTH, K_COUNT };
+  enum { K_ANALYSIS, K_FILTER, K_SYNTH, K_SWEEP, K_COMBINE, K_COUNT };
   struct Rec {
     int kind;
     hipEvent_t a, b;
@@ -144,6 +164,8 @@ struct fft3d_plane {
     (void)hipFree(dStage);
     (void)hipFree(dBlocks);
     (void)hipFree(dKalman);
+    (void)hipFree(dPart);
+    (void)hipFree(dSeams);
     if (own) (void)hipStreamDestroy(own);
     g_bytes -= bytes;
   }
@@ -184,6 +206,60 @@ struct fft3d_plane {
       bytes -= specBytes;
       g_bytes -= specBytes;
     }
+  }
+
+  // Chooses the band height and rebuilds the border-tile list for it.
+  bool layoutSweep() {
+    const int nbw = ks.nbw;
+    const int nstrips = (ncols + nbw - 1) / nbw;
+    int R = bandRows;
+    if (R <= 0) {
+      // about two workgroups per CU on a 256-CU part, at least 2 rows per band
+      int nbands = (512 + nstrips - 1) / nstrips;
+      R = (nrows + nbands - 1) / nbands;
+      if (R < 2) R = 2;
+    }
+    if (R > nrows) R = nrows;
+    if (R == seamR && dSeams) return true;
+    const int nbands = (nrows + R - 1) / R;
+    const int sx = p.bw / 2, sy = p.bh / 2;
+    const int ntx = (d.width + sx - 1) / sx, nty = (d.height + sy - 1) / sy;
+    std::vector<char> rowIn(nty, 0), colIn(ntx, 0);
+    for (int i = 0; i < nbands; i++)
+      for (int u = i * R; u <= std::min(i * R + R, nrows) - 2; u++)
+        if (u >= 0 && u < nty) rowIn[u] = 1;
+    for (int st = 0; st < nstrips; st++) {
+      const int nlive = std::min(nbw, ncols - st * nbw);
+      for (int t = st * nbw; t <= st * nbw + nlive - 2; t++)
+        if (t >= 0 && t < ntx) colIn[t] = 1;
+    }
+    std::vector<int2> seams;
+    for (int u = 0; u < nty; u++)
+      for (int t = 0; t < ntx; t++)
+        if (!rowIn[u] || !colIn[t]) seams.push_back(make_int2(t, u));
+    if (dSeams) {
+      (void)hipFree(dSeams);
+      bytes -= (size_t)nseams * sizeof(int2);
+      g_bytes -= (size_t)nseams * sizeof(int2);
+      dSeams = nullptr;
+    }
+    nseams = (int)seams.size();
+    if (nseams && (!alloc(reinterpret_cast<void**>(&dSeams), seams.size() * sizeof(int2)) ||
+                   hipMemcpy(dSeams, seams.data(), seams.size() * sizeof(int2),
+                             hipMemcpyHostToDevice) != hipSuccess)) {
+      (void)hipGetLastError();
+      return false;
+    }
+    seamR = R;
+    sw.R = R;
+    sw.nstrips = nstrips;
+    sw.nbands = nbands;
+    sw.ntx = ntx;
+    sw.nty = nty;
+    sw.part = dPart;
+    sw.seams = dSeams;
+    sw.nseams = nseams;
+    return true;
   }
 
   int fail(int code, const std::string& why) {
@@ -300,9 +376,14 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
       chromaNorm = 32768.0f / 65535.0f;
   }
 
-  // one device table block: 3 windows, pattern, sharpen, degrid
+  // one device table block: 3 windows, pattern, sharpen, degrid, separable factors
   const size_t nwin = (size_t)bw * bh, nbin = (size_t)pl->fbw * bh;
-  std::vector<float> host(3 * nwin + 4 * nbin, 0.0f);
+  const size_t sep = 3 * nwin + 4 * nbin;
+  std::vector<float> host(sep + 2 * bw + 2 * bh, 0.0f);
+  std::copy(pl->t.anaX.begin(), pl->t.anaX.end(), host.begin() + sep);
+  std::copy(pl->t.anaY.begin(), pl->t.anaY.end(), host.begin() + sep + bw);
+  std::copy(pl->t.synX.begin(), pl->t.synX.end(), host.begin() + sep + bw + bh);
+  std::copy(pl->t.synY.begin(), pl->t.synY.end(), host.begin() + sep + 2 * bw + bh);
   std::copy(pl->t.winAna.begin(), pl->t.winAna.end(), host.begin());
   std::copy(pl->t.winSynA.begin(), pl->t.winSynA.end(), host.begin() + nwin);
   std::copy(pl->t.winSynB.begin(), pl->t.winSynB.end(), host.begin() + 2 * nwin);
@@ -316,6 +397,11 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   ok = ok && pl->alloc(reinterpret_cast<void**>(&pl->dBlocks), sizeof(float) * (size_t)pl->nb * bw * bh);
   if (ok && p.bm == 0)
     ok = pl->alloc(reinterpret_cast<void**>(&pl->dKalman), sizeof(float4) * (size_t)pl->nb * nbin);
+  pl->sweepOk = p.mode == 1 && p.ow * 2 == bw && p.oh * 2 == bh;
+  if (ok && pl->sweepOk) {
+    ok = pl->alloc(reinterpret_cast<void**>(&pl->dPart), sizeof(float) * 4 * (size_t)W * H);
+    ok = ok && pl->layoutSweep();
+  }
   if (!ok) {
     (void)hipGetLastError();
     delete pl;
@@ -368,6 +454,10 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   a.pattern = pl->dTables + 3 * nwin;
   a.sharpen = pl->dTables + 3 * nwin + nbin;
   a.gs = reinterpret_cast<const fk::c32*>(pl->dTables + 3 * nwin + 2 * nbin);
+  a.anaX = pl->dTables + sep;
+  a.anaY = pl->dTables + sep + bw;
+  a.synX = pl->dTables + sep + bw + bh;
+  a.synY = pl->dTables + sep + 2 * bw + bh;
   *out = pl;
   return FFT3D_OK;
 }
@@ -433,21 +523,41 @@ int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst, 
     if (it == p->ring.end()) return p->fail(FFT3D_EMISSING, "missing spectra in window");
     w.s[i] = it->second;
   }
+  const int kfirst = (p->p.bm == 0 && p->kalmanFirst) ? 1 : 0;
+  void* outp = dst_on_device ? dst : p->dStage;
+  if (p->sweepOk && p->sweepOn) {
+    if (!p->layoutSweep()) return p->fail(FFT3D_ENOMEM, "GPU alloc failed (seam list)");
+    fk::SweepArgs a = p->sw;
+    a.out = outp;
+    SweepFn ksw = p->ks.sweep;
+    hipEvent_t t0 = p->prof_begin(s);
+    FK_LAUNCH(ksw, dim3(a.nstrips, a.nbands), dim3(p->ks.sweepThreads), s, p->args, w,
+              p->dKalman, kfirst, a);
+    p->prof_end(t0, fft3d_plane::K_SWEEP, s);
+    if (!p->launched("filter/synthesis sweep")) return FFT3D_ELAUNCH;
+    if (p->p.bm == 0) p->kalmanFirst = false;
+    if (a.nseams) {
+      CombineFn kc = &fk::k_combine_seams;
+      t0 = p->prof_begin(s);
+      FK_LAUNCH(kc, dim3(a.nseams), dim3(256), s, p->args, a, p->ks.nbw);
+      p->prof_end(t0, fft3d_plane::K_COMBINE, s);
+      if (!p->launched("seam combine")) return FFT3D_ELAUNCH;
+    }
+  } else {
   const int grid = (p->nb + p->ks.G - 1) / p->ks.G;
   FilterFn kf = p->ks.filter_inverse;
-  const int kfirst = (p->p.bm == 0 && p->kalmanFirst) ? 1 : 0;
   hipEvent_t t0 = p->prof_begin(s);
   FK_LAUNCH(kf, dim3(grid), dim3(64), s, p->args, w, p->dKalman, kfirst, p->dBlocks);
   p->prof_end(t0, fft3d_plane::K_FILTER, s);
   if (!p->launched("filter/inverse")) return FFT3D_ELAUNCH;
   if (p->p.bm == 0) p->kalmanFirst = false;
-  void* outp = dst_on_device ? dst : p->dStage;
   SynthFn ksy = &fk::k_synthesize;
   t0 = p->prof_begin(s);
   FK_LAUNCH(ksy, dim3((p->d.width + 15) / 16, (p->d.height + 15) / 16), dim3(256), s, p->args,
             p->dBlocks, outp);
   p->prof_end(t0, fft3d_plane::K_SYNTH, s);
   if (!p->launched("synthesize")) return FFT3D_ELAUNCH;
+  }
   if (!dst_on_device) {
     if (hipMemcpyAsync(dst, p->dStage, need, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -477,6 +587,21 @@ void fft3d_plane_clear(fft3d_plane* p) {
   p->ring.clear();
 }
 
+int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
+  if (!p) return FFT3D_EINVAL;
+  switch (option) {
+  case FFT3D_OPT_SWEEP:
+    if (value != 0 && value != 1) return p->fail(FFT3D_EINVAL, "sweep option must be 0 or 1");
+    p->sweepOn = value == 1;
+    return FFT3D_OK;
+  case FFT3D_OPT_BAND_ROWS:
+    if (value < 0) return p->fail(FFT3D_EINVAL, "band rows must be >= 0");
+    p->bandRows = value;
+    return FFT3D_OK;
+  }
+  return p->fail(FFT3D_EINVAL, "unknown option");
+}
+
 int fft3d_plane_set_profiling(fft3d_plane* p, int on) {
   if (!p) return FFT3D_EINVAL;
   p->prof = on != 0;
@@ -485,9 +610,10 @@ int fft3d_plane_set_profiling(fft3d_plane* p, int on) {
 
 int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max) {
   if (!p) return FFT3D_EINVAL;
-  static const char* names[fft3d_plane::K_COUNT] = {"analysis", "filter_inverse", "synthesize"};
-  double ms[fft3d_plane::K_COUNT] = {0, 0, 0};
-  int n[fft3d_plane::K_COUNT] = {0, 0, 0};
+  static const char* names[fft3d_plane::K_COUNT] = {"analysis", "filter_inverse", "synthesize",
+                                                     "filter_synth_sweep", "combine_seams"};
+  double ms[fft3d_plane::K_COUNT] = {};
+  int n[fft3d_plane::K_COUNT] = {};
   for (auto& r : p->recs) {
     float t = 0.0f;
     if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {

@@ -136,6 +136,10 @@ struct Tables {
   std::vector<float> pattern;                   // fbw*bh, empty when off
   std::vector<float> sharpen;                   // fbw*bh, empty when off
   std::vector<float> degrid;                    // 2*fbw*bh (re,im), empty when off
+  // Mode 1 windows are separable: winAna = anaX[lx]*anaY[ly] and, at half
+  // overlap, winSynA = winSynB = synX[lx]*synY[ly]/(bw*bh) (the inverse
+  // transform's normalisation folded into synY). Empty in mode 0.
+  std::vector<float> anaX, anaY, synX, synY;
 };
 
 namespace detail {
@@ -199,6 +203,11 @@ inline Tables buildTables(const Params& p) {
     std::vector<double> wxSB = profile(bw, p.ow, syx, true);
     std::vector<double> wyA = profile(bh, p.oh, any, false);
     std::vector<double> wySA = profile(bh, p.oh, syy, false);
+    t.anaX.assign(wxA.begin(), wxA.end());
+    t.anaY.assign(wyA.begin(), wyA.end());
+    t.synX.assign(wxSA.begin(), wxSA.end());
+    t.synY.resize(bh);
+    for (int ly = 0; ly < bh; ly++) t.synY[ly] = (float)(wySA[ly] / ((double)bw * bh));
     for (int ly = 0; ly < bh; ly++)
       for (int lx = 0; lx < bw; lx++) {
         t.winAna[ly * bw + lx] = (float)(wxA[lx] * wyA[ly]);

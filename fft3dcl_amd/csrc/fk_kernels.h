@@ -49,6 +49,12 @@ struct PlaneArgs {
   const float* pattern; // (bw/2+1)*bh sigma pattern (pat)
   const float* sharpen; // (bw/2+1)*bh sharpen gains (shp)
   const c32* gs;        // (bw/2+1)*bh degrid sample (dg)
+  // Separable factors of the mode-1 windows (window = fx[lx] * fy[ly]); the
+  // synthesis factors already include the inverse transform's 1/(bw*bh).
+  const float* anaX;   // bw
+  const float* anaY;   // bh
+  const float* synX;   // bw (half overlap: both block parities share it)
+  const float* synY;   // bh
 };
 
 // The temporal window handed to the filter: frames in temporal order.
@@ -121,12 +127,27 @@ __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* pl
     block_origin(P, b, x0, y0, par);
     float zr[BW], zi[BW];
     const int r0 = 2 * j;
-    const float* w0 = P.winAna + r0 * BW;
-    const float* w1 = w0 + BW;
 #pragma unroll
     for (int k = 0; k < BW; k++) {
-      zr[k] = (fetch(P, plane, x0 + k, y0 + r0) - P.chromaNorm) * w0[k];
-      zi[k] = (fetch(P, plane, x0 + k, y0 + r0 + 1) - P.chromaNorm) * w1[k];
+      zr[k] = fetch(P, plane, x0 + k, y0 + r0) - P.chromaNorm;
+      zi[k] = fetch(P, plane, x0 + k, y0 + r0 + 1) - P.chromaNorm;
+    }
+    if (P.mode == 1) {
+      // separable window: a uniform column factor times a per-lane row factor
+      const float fy0 = P.anaY[r0], fy1 = P.anaY[r0 + 1];
+#pragma unroll
+      for (int k = 0; k < BW; k++) {
+        const float fx = P.anaX[k];
+        zr[k] *= fx * fy0;
+        zi[k] *= fx * fy1;
+      }
+    } else {
+      const float* w0 = P.winAna + r0 * BW;
+#pragma unroll
+      for (int k = 0; k < BW; k++) {
+        zr[k] *= w0[k];
+        zi[k] *= w0[BW + k];
+      }
     }
     fft_reg<BW, false>(zr, zi);
     c32* Z = lds + (g * S::NROW + j) * S::ZP;
@@ -381,7 +402,7 @@ __device__ __forceinline__ void repack_block(bool live, int lb, c32* Yg) {
 
 // Inverse 2-D C2R of block columns 0..NCOL-1 held in the LDS column buffer
 // (column 0 packed). On return lanes j < NROW hold real rows 2j (xr) and
-// 2j+1 (xi), scaled by 1/(bw*bh). All lanes call.
+// 2j+1 (xi), unnormalised (x bw*bh). All lanes call.
 template <int BW, int BH>
 __device__ __forceinline__ void inverse_block(bool live, int j, c32* Yg, float (&xr)[BW],
                                               float (&xi)[BW]) {
@@ -417,12 +438,6 @@ __device__ __forceinline__ void inverse_block(bool live, int j, c32* Yg, float (
       xi[BW - c] = -a.y + q.x;
     }
     fft_reg<BW, true>(xr, xi);
-    const float sc = 1.0f / (float)(BW * BH);
-#pragma unroll
-    for (int k = 0; k < BW; k++) {
-      xr[k] *= sc;
-      xi[k] *= sc;
-    }
   }
 }
 
@@ -447,10 +462,11 @@ __global__ __launch_bounds__(64) void k_filter_inverse(PlaneArgs P, WindowArgs W
     block_origin(P, b, x0, y0, par);
     const float* win = par ? P.winSynB : P.winSynA;
     float* dst = blocks + (size_t)b * BW * BH + (2 * j) * BW;
+    const float sc = 1.0f / (float)(BW * BH);
 #pragma unroll
     for (int k = 0; k < BW; k++) {
-      dst[k] = xr[k] * win[2 * j * BW + k];
-      dst[BW + k] = xi[k] * win[(2 * j + 1) * BW + k];
+      dst[k] = (xr[k] * sc) * win[2 * j * BW + k];
+      dst[BW + k] = (xi[k] * sc) * win[(2 * j + 1) * BW + k];
     }
   }
 }

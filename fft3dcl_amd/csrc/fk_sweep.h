@@ -1,0 +1,189 @@
+// Fused filter -> inverse FFT -> windowed overlap-add -> pack (mode 1, half
+// overlap: ow = bw/2, oh = bh/2, every output pixel covered by 2x2 blocks).
+//
+// Replaces the reference's fft2d_inv + synthesize pair (src/cl/kernels.h:
+// 368-448) and its `blocks` round trip through device memory (nb*bw*bh
+// floats written and read again per frame).
+//
+// Tiling. Output tiles are (bw/2) x (bh/2); block (m, n) covers tiles
+// (m-1..m) x (n-1..n). A workgroup of WPW waves owns a strip of NBW = WPW*G
+// adjacent block columns and a band of R block rows, and sweeps down the
+// band one block row at a time:
+//   1. each wave filters + inverse-transforms its G blocks (fk_kernels.h);
+//   2. the windowed blocks are overlap-added along the row in an LDS strip,
+//      even block columns storing and odd ones adding (fixed order: the
+//      result is bit-identical run to run, no atomics);
+//   3. the strip's upper half plus the lower half carried in registers from
+//      the previous block row completes one tile row: interior tiles are
+//      packed and stored to the output plane; tiles on the band's or strip's
+//      border are stored as float partial sums into one of four partial
+//      planes chosen by the (band, strip) parity, so neighbours never write
+//      the same word;
+//   4. k_combine_seams then adds the partials of those border tiles in a
+//      fixed (band, strip) order and packs them.
+#pragma once
+
+#include "fk_kernels.h"
+
+namespace fk {
+
+struct SweepArgs {
+  int R;             // block rows per band
+  int nstrips, nbands;
+  int ntx, nty;      // output tiles covering the plane
+  float* part;       // 4 partial planes of W*H floats
+  void* out;         // packed output plane (u8/u16/f32)
+  const int2* seams; // border tiles (tile column, tile row)
+  int nseams;
+};
+
+template <int BW, int BH, int WPW>
+struct SweepShape {
+  using S = Shape<BW, BH>;
+  static constexpr int SX = BW / 2, SY = BH / 2;  // tile size
+  static constexpr int NBW = S::G * WPW;          // block columns per strip
+  static constexpr int CW = (NBW + 1) * SX;       // strip width in pixels
+  static constexpr int SP = CW + 1;               // strip pitch in floats
+  static constexpr int NT = 64 * WPW;             // threads per workgroup
+  static constexpr int NCARRY = (SY * CW + NT - 1) / NT;
+  static constexpr int YFLOATS = 2 * S::G * S::FBW * S::YP * WPW;
+  static constexpr int LDS_FLOATS = YFLOATS > BH * SP ? YFLOATS : BH * SP;
+};
+
+template <int BW, int BH, int WPW>
+__global__ __launch_bounds__(64 * WPW) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
+                                                           float4* kstate, int kfirst,
+                                                           SweepArgs A) {
+  using S = Shape<BW, BH>;
+  using T = SweepShape<BW, BH, WPW>;
+  constexpr int SX = T::SX, SY = T::SY, NBW = T::NBW, CW = T::CW, SP = T::SP;
+  __shared__ float lds[T::LDS_FLOATS];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int g = lane / S::LPB, j = lane - g * S::LPB;
+  const int strip = blockIdx.x, band = blockIdx.y;
+  const int n0 = strip * NBW, m0 = band * A.R;
+  const int mEnd = min(m0 + A.R, P.nrows);
+  const int nlive = min(NBW, P.ncols - n0);
+  const int n = n0 + w * S::G + g;  // this lane's block column
+  const bool colLive = n < P.ncols;
+  c32* Yg = reinterpret_cast<c32*>(lds) + (w * S::G + g) * S::FBW * S::YP;
+  float* strp = lds;
+  float* part = A.part + (size_t)((band & 1) * 2 + (strip & 1)) * P.W * P.H;
+  const int validCols = (nlive + 1) * SX;
+  const int x0 = (n0 - 1) * SX;  // image column of strip column 0
+  float carry[T::NCARRY];
+
+  for (int m = m0; m < mEnd; m++) {
+    const int b = m * P.ncols + n;
+    __syncthreads();  // the previous row's strip reads are done: LDS is the Y buffers again
+    if (colLive) filter_block<BW, BH>(P, Wn, b, j, Yg, kstate, kfirst != 0);
+    __syncthreads();
+    repack_block<BW, BH>(colLive, j, Yg);
+    float xr[BW], xi[BW];
+    inverse_block<BW, BH>(colLive, j, Yg, xr, xi);
+    const bool rowLane = colLive && j < S::NROW;
+    if (rowLane) {
+      // half overlap: both parities use the same separable synthesis window
+      const float fy0 = P.synY[2 * j], fy1 = P.synY[2 * j + 1];
+#pragma unroll
+      for (int k = 0; k < BW; k++) {
+        const float fx = P.synX[k];
+        xr[k] *= fx * fy0;
+        xi[k] *= fx * fy1;
+      }
+    }
+    __syncthreads();  // every wave is done with its Y buffer: LDS becomes the strip
+    const int sc = (n - n0) * SX;
+    if (rowLane && (n & 1) == 0) {
+      float* r0 = strp + (2 * j) * SP + sc;
+#pragma unroll
+      for (int k = 0; k < BW; k++) {
+        r0[k] = xr[k];
+        r0[SP + k] = xi[k];
+      }
+    }
+    __syncthreads();
+    if (rowLane && (n & 1) == 1) {
+      // left half always overlaps the even neighbour n-1 (same strip); the
+      // right half overlaps n+1 only when that block exists in the strip
+      const bool rightAdd = (n + 1 < n0 + nlive);
+      float* r0 = strp + (2 * j) * SP + sc;
+#pragma unroll
+      for (int k = 0; k < BW; k++) {
+        if (k < SX || rightAdd) {
+          r0[k] += xr[k];
+          r0[SP + k] += xi[k];
+        } else {
+          r0[k] = xr[k];
+          r0[SP + k] = xi[k];
+        }
+      }
+    }
+    __syncthreads();
+    // tile row m-1: carried lower half of row m-1 + upper half of row m
+    const int y0 = (m - 1) * SY;
+#pragma unroll
+    for (int i = 0; i < T::NCARRY; i++) {
+      const int p = tid + T::NT * i;
+      if (p < SY * CW) {
+        const int r = p / CW, c = p - r * CW;
+        const bool ok = c < validCols;
+        const float top = ok ? strp[r * SP + c] : 0.0f;
+        const float v = (m == m0) ? top : carry[i] + top;
+        const int x = x0 + c, y = y0 + r;
+        if (ok && x >= 0 && x < P.W && y >= 0 && y < P.H) {
+          const bool border = (m == m0) || c < SX || c >= nlive * SX;
+          const size_t idx = (size_t)y * P.W + x;
+          if (border)
+            part[idx] = v;
+          else
+            store_sample(P, A.out, idx, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
+        }
+        carry[i] = ok ? strp[(r + SY) * SP + c] : 0.0f;
+      }
+    }
+  }
+  // tile row mEnd-1: lower half of the band's last block row (a border row)
+  const int y0 = (mEnd - 1) * SY;
+#pragma unroll
+  for (int i = 0; i < T::NCARRY; i++) {
+    const int p = tid + T::NT * i;
+    if (p < SY * CW) {
+      const int r = p / CW, c = p - r * CW;
+      const int x = x0 + c, y = y0 + r;
+      if (c < validCols && x >= 0 && x < P.W && y >= 0 && y < P.H)
+        part[(size_t)y * P.W + x] = carry[i];
+    }
+  }
+}
+
+// Adds the partial sums of one border tile from every workgroup that touched
+// it, in (band, strip) order, adds chroma centring and packs.
+__global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A, int nbw) {
+  const int2 tile = A.seams[blockIdx.x];
+  const int sx = P.bw / 2, sy = P.bh / 2;
+  // bands touching tile row u: band i covers tile rows [i*R - 1, i*R + R - 1]
+  int bands[2], nb = 0;
+  for (int i = (tile.y) / A.R; i <= (tile.y + 1) / A.R && nb < 2; i++)
+    if (i < A.nbands && i * A.R - 1 <= tile.y && tile.y <= min(i * A.R + A.R, P.nrows) - 1)
+      if (nb == 0 || bands[nb - 1] != i) bands[nb++] = i;
+  int strips[2], ns = 0;
+  for (int s = tile.x / nbw; s <= (tile.x + 1) / nbw && ns < 2; s++) {
+    if (s >= A.nstrips) continue;
+    const int nlive = min(nbw, P.ncols - s * nbw);
+    if (s * nbw - 1 <= tile.x && tile.x <= s * nbw + nlive - 1)
+      if (ns == 0 || strips[ns - 1] != s) strips[ns++] = s;
+  }
+  for (int q = threadIdx.x; q < sx * sy; q += blockDim.x) {
+    const int x = tile.x * sx + q % sx, y = tile.y * sy + q / sx;
+    if (x >= P.W || y >= P.H) continue;
+    const size_t idx = (size_t)y * P.W + x;
+    float v = 0.0f;
+    for (int a = 0; a < nb; a++)
+      for (int c = 0; c < ns; c++)
+        v += A.part[(size_t)((bands[a] & 1) * 2 + (strips[c] & 1)) * P.W * P.H + idx];
+    store_sample(P, A.out, idx, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
+  }
+}
+
+}  // namespace fk

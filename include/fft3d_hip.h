@@ -138,6 +138,17 @@ int fft3d_plane_set_profiling(fft3d_plane* p, int on);
  * last call (up to max entries) and resets them. */
 int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max);
 
+/* Execution options of a plane (tests and tuning; results never depend on
+ * them beyond floating-point summation order):
+ *   FFT3D_OPT_SWEEP      1 (default): fused filter/inverse/overlap-add sweep
+ *                        where the geometry allows it (mode 1, ow = bw/2,
+ *                        oh = bh/2); 0: separate filter/inverse and
+ *                        synthesis kernels through a blocks buffer.
+ *   FFT3D_OPT_BAND_ROWS  block rows per sweep band (0 = automatic).
+ * Returns FFT3D_EINVAL for an unknown option or value. */
+enum { FFT3D_OPT_SWEEP = 1, FFT3D_OPT_BAND_ROWS = 2 };
+int fft3d_plane_set_option(fft3d_plane* p, int option, int value);
+
 /* Introspection for tests and tools. */
 int fft3d_plane_get_info(const fft3d_plane* p, fft3d_plane_info* info);
 /* Host copy of a constant table: 0 analysis window, 1/2 synthesis windows

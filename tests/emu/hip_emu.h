@@ -8,6 +8,7 @@
 // private to the running workgroup. "Device" memory is host memory.
 #pragma once
 
+#include <algorithm>
 #include <barrier>
 #include <cmath>
 #include <cstdint>
@@ -32,6 +33,12 @@ struct float4 {
   float x, y, z, w;
 };
 inline float4 make_float4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
+struct int2 {
+  int x, y;
+};
+inline int2 make_int2(int x, int y) { return int2{x, y}; }
+using std::max;
+using std::min;
 
 inline thread_local dim3 threadIdx, blockIdx;
 inline dim3 blockDim, gridDim;

@@ -176,3 +176,34 @@ def test_emu_ring_prune_and_counts(emu_lib):
     assert capi.live_instances(emu_lib) == inst_before + 1
     inst.close()
     assert capi.live_instances(emu_lib) == inst_before
+
+
+@pytest.mark.parametrize("kind,kw,W,H,R", [
+    ("u8", dict(bt=3), 600, 100, 2),                      # 3 strips x 4 bands
+    ("u10", dict(bt=4, sharpen=0.4), 530, 90, 3),         # ragged last strip and band
+    ("f32", dict(bt=0, sigma2=3.0, sigma3=2.0, sigma4=1.0), 300, 70, 0),
+    ("u8", dict(bt=2, bw=16, bh=16), 700, 60, 5),
+    ("u8", dict(bt=3, bw=8, bh=8), 530, 40, 0),
+    ("u16", dict(bt=1, bw=64, bh=64), 600, 200, 2),
+    ("u8", dict(bt=3, bw=64, bh=32), 700, 100, 1),
+    ("u8", dict(bt=-1, bw=4, bh=4), 600, 24, 3),
+])
+def test_emu_sweep_seams_match_oracle(oracle, emu_lib, kind, kw, W, H, R):
+    from fft3dcl_amd.filter import Instance
+    bits, st = kind_info(kind)
+    planes = make_clip(6, W, H, 4, kind)
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st)
+    outs = {}
+    for sweep in (1, 0):
+        inst = Instance(1, W, H, bits, st, 4, lib=emu_lib, **kw)
+        inst.objs[0].set_option(capi.Plane.OPT_SWEEP, sweep)
+        inst.objs[0].set_option(capi.Plane.OPT_BAND_ROWS, R)
+        out = np.zeros_like(planes[0])
+        for n in range(4):
+            inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
+        inst.close()
+        outs[sweep] = out
+        if kind == "f32":
+            assert max_abs(out, want[0]) < 1e-5
+        else:
+            assert lsb_diff(out, want[0]) <= 1

@@ -155,6 +155,41 @@ def test_profiling_counts_launches(gpu):
     for k in range(5):
         inst.get_frame(k, [lambda fn: np.ascontiguousarray(frames[fn])], [out])
     prof = inst.objs[0].profile()
-    assert prof["filter_inverse"][0] == 5 and prof["analysis"][0] == 5
+    # the fused sweep (+ seam combine) is the default path at half overlap
+    assert prof["filter_synth_sweep"][0] == 5 and prof["analysis"][0] == 5
+    assert prof["combine_seams"][0] == 5
     assert all(ms > 0 for _, ms in prof.values())
+    inst.objs[0].set_option(capi.Plane.OPT_SWEEP, 0)
+    for k in range(5):
+        inst.get_frame(k, [lambda fn: np.ascontiguousarray(frames[fn])], [out])
+    prof = inst.objs[0].profile()
+    assert prof["filter_inverse"][0] == 5 and prof["synthesize"][0] == 5
     inst.close()
+
+
+@pytest.mark.parametrize("kind,kw,W,H,R", [
+    ("u10", dict(bt=4, sharpen=0.4, sigma=3.0), 1920, 1080, 0),
+    ("u8", dict(bt=3), 1000, 700, 3),
+    ("f32", dict(bt=0, bw=64, bh=64), 1300, 700, 2),
+    ("u8", dict(bt=2, bw=16, bh=16), 900, 500, 0),
+    ("u8", dict(bt=3, bw=8, bh=8), 700, 300, 4),
+])
+def test_sweep_equals_unfused_path(gpu, oracle, kind, kw, W, H, R):
+    """The fused sweep (in-LDS overlap-add, border tiles through partial
+    planes) against the unfused filter/inverse + synthesis kernels and the
+    oracle, on frames with several strips and bands."""
+    bits, st = kind_info(kind)
+    planes = make_clip(13, W, H, 4, kind)
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st)
+    outs = []
+    for sweep in (1, 0):
+        inst = Instance(1, W, H, bits, st, 4, lib=gpu, **kw)
+        inst.objs[0].set_option(capi.Plane.OPT_SWEEP, sweep)
+        inst.objs[0].set_option(capi.Plane.OPT_BAND_ROWS, R)
+        out = np.zeros_like(planes[0])
+        for n in range(4):
+            inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
+        inst.close()
+        check(kind, out, want[0])
+        outs.append(out)
+    check(kind, outs[0], outs[1])
