@@ -2,7 +2,7 @@
 //
 // Test infrastructure: lets the kernels and the C-ABI host run on a machine
 // with no GPU, so index arithmetic, LDS layouts and barriers are checked
-// (under AddressSanitizer in tests/test_emu_kernels.py) before a kernel ever
+// (under AddressSanitizer in tests/test_emu_asan.py) before a kernel ever
 // runs on the device. One OS thread per lane; workgroups run one after the
 // other so a kernel's __shared__ arrays (function-local statics here) are
 // private to the running workgroup. "Device" memory is host memory.

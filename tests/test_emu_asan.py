@@ -1,0 +1,50 @@
+"""Runs a set of emulated kernel launches (tests/emu, thread-per-lane CPU
+emulation of the HIP kernels) under AddressSanitizer in a child process, so an
+out-of-bounds global or LDS access in the kernels' index arithmetic fails on
+the CPU before it could fault a GPU. LDS arrays are function statics in the
+emulation, so their overruns are caught as global-buffer overflows."""
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+ASAN_LIB = ROOT / "tests" / "_build" / "asan" / "libfft3d_emu.so"
+
+SCRIPT = r"""
+import sys, numpy as np
+sys.path.insert(0, {root!r}); sys.path.insert(0, {tests!r})
+from fft3dcl_amd.filter import FFT3DCL, Instance
+from fft3dcl_amd.synth import make_clip
+from fft3dcl_amd import capi
+lib = {lib!r}
+cases = [("u8", 8, 0, dict(bt=3)), ("u10", 10, 0, dict(bt=4, sharpen=0.4)),
+         ("f32", 32, 1, dict(bt=0, bw=64, bh=64)), ("u8", 8, 0, dict(bt=2, bw=8, bh=16)),
+         ("u8", 8, 0, dict(bt=-1, bw=4, bh=4)), ("u8", 8, 0, dict(bt=3, mode=0)),
+         ("u16", 16, 0, dict(bt=1, bw=32, bh=16, ow=6, oh=4))]
+for kind, bits, st, kw in cases:
+    for W, H in ((83, 45), (300, 70)):
+        planes = make_clip(1, W, H, 3, kind)
+        for sweep in (1, 0):
+            inst = Instance(1, W, H, bits, st, 3, lib=lib, **kw)
+            inst.objs[0].set_option(1, sweep)
+            out = np.zeros_like(planes[0])
+            for n in range(3):
+                inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
+            inst.close()
+print("asan-ok")
+"""
+
+
+def test_emulated_kernels_under_asan():
+    if not ASAN_LIB.exists():
+        pytest.skip("ASan emulation build missing (make -C tests/emu)")
+    libasan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True,
+                             text=True).stdout.strip()
+    env = dict(os.environ, LD_PRELOAD=libasan, ASAN_OPTIONS="detect_leaks=0")
+    code = SCRIPT.format(root=str(ROOT), tests=str(ROOT / "tests"), lib=str(ASAN_LIB))
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=900)
+    assert r.returncode == 0 and "asan-ok" in r.stdout, (r.stdout[-2000:] + r.stderr[-4000:])
