@@ -333,40 +333,65 @@ __device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArg
                                              int lb, c32* Yg, float4* kstate, bool kfirst) {
   using S = Shape<BW, BH>;
   constexpr int FBW = S::FBW, NCOL = S::NCOL, T = BH * FBW;
+  constexpr int NI = (T + S::LPB - 1) / S::LPB;  // bins per lane
+  constexpr int U = 4;  // bins whose loads are in flight together
   const size_t base = (size_t)b * S::BSTR;
   const int nwin = Wn.nwin;
   // block DC of the current frame: X[0][0] = Re W[0]
   const float dc = P.dg ? Wn.s[Wn.cur][base].x : 0.0f;
   float4* kb = kstate ? kstate + (size_t)b * T : nullptr;
 #pragma unroll 1
-  for (int k = lb; k < T; k += S::LPB) {
-    int r, col;
-    if (k < BH * NCOL) {
-      r = k / NCOL;
-      col = k - r * NCOL;
-    } else {
-      r = k - BH * NCOL;
-      col = NCOL;
-    }
-    const int r2 = (BH - r) & (BH - 1);
-    c32 v[4];
+  for (int i0 = 0; i0 < NI; i0 += U) {
+    c32 va[U][4], vb[U][4];
+    int rr[U], cc[U];
+    bool ok[U];
+    // issue every load of U bins before any arithmetic
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      v[t] = cmake(0.0f, 0.0f);
-      if (t < nwin) {
-        if (col != 0 && col != NCOL) {
-          v[t] = Wn.s[t][base + r * NCOL + col];
+    for (int u = 0; u < U; u++) {
+      const int k = lb + (i0 + u) * S::LPB;
+      ok[u] = (i0 + u < NI) && k < T;
+      int r = 0, col = 1;
+      if (ok[u]) {
+        if (k < BH * NCOL) {
+          r = k / NCOL;
+          col = k - r * NCOL;
         } else {
-          // X[r][0] = (W[r] + conj W[-r]) / 2, X[r][h] = (W[r] - conj W[-r]) / 2i
-          const c32 wa = Wn.s[t][base + r * NCOL], wb = Wn.s[t][base + r2 * NCOL];
-          v[t] = col == 0 ? cmake(0.5f * (wa.x + wb.x), 0.5f * (wa.y - wb.y))
-                          : cmake(0.5f * (wa.y + wb.y), -0.5f * (wa.x - wb.x));
+          r = k - BH * NCOL;
+          col = NCOL;
+        }
+      }
+      rr[u] = r;
+      cc[u] = col;
+      const bool pk = col == 0 || col == NCOL;  // bin of the packed DC/Nyquist pair
+      const size_t ia = base + r * NCOL + (pk ? 0 : col);
+      const size_t ib = base + ((BH - r) & (BH - 1)) * NCOL;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        va[u][t] = cmake(0.0f, 0.0f);
+        vb[u][t] = cmake(0.0f, 0.0f);
+        if (t < nwin && ok[u]) {
+          va[u][t] = Wn.s[t][ia];
+          if (pk) vb[u][t] = Wn.s[t][ib];
         }
       }
     }
-    const BinConst kc = bin_const<BW>(P, r, col);
-    Yg[col * S::YP + r] = filter_bin(P, v, cscale(kc.gsv, dc), kc.sX, kc.sY, kc.lut,
-                                     kc.kcn, kb ? kb + r * FBW + col : nullptr, kfirst);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!ok[u]) continue;
+      const int r = rr[u], col = cc[u];
+      c32 v[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const c32 wa = va[u][t], wb = vb[u][t];
+        // X[r][0] = (W[r] + conj W[-r]) / 2, X[r][h] = (W[r] - conj W[-r]) / 2i
+        v[t] = col == 0      ? cmake(0.5f * (wa.x + wb.x), 0.5f * (wa.y - wb.y))
+               : col == NCOL ? cmake(0.5f * (wa.y + wb.y), -0.5f * (wa.x - wb.x))
+                             : wa;
+      }
+      const BinConst kc = bin_const<BW>(P, r, col);
+      Yg[col * S::YP + r] = filter_bin(P, v, cscale(kc.gsv, dc), kc.sX, kc.sY, kc.lut,
+                                       kc.kcn, kb ? kb + r * FBW + col : nullptr, kfirst);
+    }
   }
 }
 

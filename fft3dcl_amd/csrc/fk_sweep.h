@@ -51,7 +51,7 @@ struct SweepShape {
 };
 
 template <int BW, int BH, int WPW>
-__global__ __launch_bounds__(64 * WPW) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
+__global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
                                                            float4* kstate, int kfirst,
                                                            SweepArgs A) {
   using S = Shape<BW, BH>;
@@ -121,38 +121,48 @@ __global__ __launch_bounds__(64 * WPW) void k_filter_synth(PlaneArgs P, WindowAr
     }
     __syncthreads();
     // tile row m-1: carried lower half of row m-1 + upper half of row m
-    const int y0 = (m - 1) * SY;
+    {
+      const int y0 = (m - 1) * SY;
+      const bool firstRow = m == m0;
+      int r = tid / CW, c = tid - (tid / CW) * CW;  // element tid + NT*i, advanced below
 #pragma unroll
-    for (int i = 0; i < T::NCARRY; i++) {
-      const int p = tid + T::NT * i;
-      if (p < SY * CW) {
-        const int r = p / CW, c = p - r * CW;
-        const bool ok = c < validCols;
-        const float top = ok ? strp[r * SP + c] : 0.0f;
-        const float v = (m == m0) ? top : carry[i] + top;
-        const int x = x0 + c, y = y0 + r;
-        if (ok && x >= 0 && x < P.W && y >= 0 && y < P.H) {
-          const bool border = (m == m0) || c < SX || c >= nlive * SX;
-          const size_t idx = (size_t)y * P.W + x;
-          if (border)
-            part[idx] = v;
-          else
-            store_sample(P, A.out, idx, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
+      for (int i = 0; i < T::NCARRY; i++) {
+        if (r < SY) {
+          const bool ok = c < validCols;
+          const float top = ok ? strp[r * SP + c] : 0.0f;
+          const float v = firstRow ? top : carry[i] + top;
+          const int x = x0 + c, y = y0 + r;
+          if (ok && x >= 0 && x < P.W && y >= 0 && y < P.H) {
+            const size_t idx = (size_t)y * P.W + x;
+            if (firstRow || c < SX || c >= nlive * SX)
+              part[idx] = v;  // border tile: partial sum
+            else
+              store_sample(P, A.out, idx, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
+          }
+          carry[i] = ok ? strp[(r + SY) * SP + c] : 0.0f;
         }
-        carry[i] = ok ? strp[(r + SY) * SP + c] : 0.0f;
+        c += T::NT;
+        while (c >= CW) {
+          c -= CW;
+          r++;
+        }
       }
     }
   }
   // tile row mEnd-1: lower half of the band's last block row (a border row)
-  const int y0 = (mEnd - 1) * SY;
+  {
+    const int y0 = (mEnd - 1) * SY;
+    int r = tid / CW, c = tid - (tid / CW) * CW;
 #pragma unroll
-  for (int i = 0; i < T::NCARRY; i++) {
-    const int p = tid + T::NT * i;
-    if (p < SY * CW) {
-      const int r = p / CW, c = p - r * CW;
+    for (int i = 0; i < T::NCARRY; i++) {
       const int x = x0 + c, y = y0 + r;
-      if (c < validCols && x >= 0 && x < P.W && y >= 0 && y < P.H)
+      if (r < SY && c < validCols && x >= 0 && x < P.W && y >= 0 && y < P.H)
         part[(size_t)y * P.W + x] = carry[i];
+      c += T::NT;
+      while (c >= CW) {
+        c -= CW;
+        r++;
+      }
     }
   }
 }
