@@ -73,9 +73,19 @@ struct Shape {
   static constexpr int FBW = BW / 2 + 1;
   static constexpr int BSTR = BH * NCOL;  // complex values stored per block
   static constexpr int ZP = BW + 1;       // LDS pitch, row-pass buffer
-  static constexpr int YP = BH + 1;       // LDS pitch, column-pass buffer
-  static constexpr int LDS_C =
-      (G * NROW * ZP > G * FBW * YP) ? G * NROW * ZP : G * FBW * YP;
+  // Per-block LDS area of the filter/inverse stage: BH rows of PITCH complex,
+  // row-major. Slots 0..NCOL-1 hold the filtered columns; while filtering, the
+  // packed DC/Nyquist column W[r] of window frame t is staged in slot st(t)
+  // (frame 0 in slot 0, which pass 2 overwrites row by row with the result,
+  // frames 1-3 in the pad slots NCOL..NCOL+2, which also break the LDS bank
+  // aliasing of the row pass).
+  static constexpr int PITCH = NCOL + 3;
+  static constexpr int YS = PITCH * BH;
+  static constexpr int LDS_C = (G * NROW * ZP > G * YS) ? G * NROW * ZP : G * YS;
+  __device__ __forceinline__ static int yidx(int c, int r) { return r * PITCH + c; }
+  __device__ __forceinline__ static int st(int t, int r) {
+    return r * PITCH + (t == 0 ? 0 : NCOL + t - 1);
+  }
 };
 
 // Direct3D MIRROR addressing (src/cpu/reference.cpp:10-17).
@@ -321,134 +331,147 @@ __device__ __forceinline__ BinConst bin_const(const PlaneArgs& P, int r, int col
   return k;
 }
 
-// Temporal filter of every bin of block b into the LDS column buffer
-// Yg[col * YP + r] (col < bw/2+1, unpacked). The BH*(BW/2+1) bins form one
-// virtual list, walked with stride LPB by the block's lanes, so every lane
-// filters the same number of bins: index k < BH*NCOL is stored bin
-// (k / NCOL, k % NCOL), with column 0 the DC column unpacked from the packed
-// pair; the BH remaining indices are the Nyquist column. All lanes of the
-// block call this with lb = lane index within the block.
+// Temporal filter of block b into its LDS area Yb (Shape::YS complex).
+//
+// Pass 1 streams the stored spectrum row by row: lanes with a column c >= 1
+// filter bin (r, c) into the swizzled column buffer; the lane holding column
+// 0 (the packed DC/Nyquist pair W[r]) stages W[r] of every window frame in
+// LDS instead. All lanes of the block read row r of every frame together, so
+// each 128-byte row line is fetched once. Every lane issues the loads of U
+// rows before any arithmetic.
+// Pass 2 (after a barrier) filters the DC and Nyquist bins from the staged
+// column: unit q pairs rows q and bh-q (unit 0 bundles the self-paired rows 0
+// and bh/2), i.e. four bins per unit and bh/2 units per block, and writes the
+// repacked W'[r] = H0[r] + i Hh[r], H = (F[r] + conj F[-r]) / 2 the Hermitian
+// part that the real inverse transform keeps.
+// Both passes are called by every lane of the block (lb = lane in block).
 template <int BW, int BH>
-__device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArgs& Wn, int b,
-                                             int lb, c32* Yg, float4* kstate, bool kfirst) {
+__device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArgs& Wn, int b,
+                                             int lb, c32* Yb, float4* kstate, bool kfirst) {
   using S = Shape<BW, BH>;
-  constexpr int FBW = S::FBW, NCOL = S::NCOL, T = BH * FBW;
-  constexpr int NI = (T + S::LPB - 1) / S::LPB;  // bins per lane
-  constexpr int U = 4;  // bins whose loads are in flight together
+  constexpr int FBW = S::FBW, NCOL = S::NCOL;
+  constexpr int RS = S::LPB / NCOL;  // lanes sharing a column split its rows
+  constexpr int NIT = BH / RS;       // rows per lane
+  constexpr int U = 4;
   const size_t base = (size_t)b * S::BSTR;
   const int nwin = Wn.nwin;
-  // block DC of the current frame: X[0][0] = Re W[0]
-  const float dc = P.dg ? Wn.s[Wn.cur][base].x : 0.0f;
-  float4* kb = kstate ? kstate + (size_t)b * T : nullptr;
+  const int c = lb % NCOL, r0 = lb / NCOL;
+  const float dc = P.dg ? Wn.s[Wn.cur][base].x : 0.0f;  // block DC: X[0][0] = Re W[0]
+  float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
 #pragma unroll 1
-  for (int i0 = 0; i0 < NI; i0 += U) {
-    c32 va[U][4], vb[U][4];
-    int rr[U], cc[U];
-    bool ok[U];
-    // issue every load of U bins before any arithmetic
+  for (int i0 = 0; i0 < NIT; i0 += U) {
+    c32 va[U][4];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int k = lb + (i0 + u) * S::LPB;
-      ok[u] = (i0 + u < NI) && k < T;
-      int r = 0, col = 1;
-      if (ok[u]) {
-        if (k < BH * NCOL) {
-          r = k / NCOL;
-          col = k - r * NCOL;
-        } else {
-          r = k - BH * NCOL;
-          col = NCOL;
-        }
-      }
-      rr[u] = r;
-      cc[u] = col;
-      const bool pk = col == 0 || col == NCOL;  // bin of the packed DC/Nyquist pair
-      const size_t ia = base + r * NCOL + (pk ? 0 : col);
-      const size_t ib = base + ((BH - r) & (BH - 1)) * NCOL;
+      const int r = r0 + (i0 + u) * RS;
 #pragma unroll
       for (int t = 0; t < 4; t++) {
         va[u][t] = cmake(0.0f, 0.0f);
-        vb[u][t] = cmake(0.0f, 0.0f);
-        if (t < nwin && ok[u]) {
-          va[u][t] = Wn.s[t][ia];
-          if (pk) vb[u][t] = Wn.s[t][ib];
-        }
+        if (t < nwin && i0 + u < NIT) va[u][t] = Wn.s[t][base + r * NCOL + c];
       }
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      if (!ok[u]) continue;
-      const int r = rr[u], col = cc[u];
+      if (i0 + u >= NIT) continue;
+      const int r = r0 + (i0 + u) * RS;
+      if (c == 0) {
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          if (t < nwin) Yb[S::st(t, r)] = va[u][t];
+      } else {
+        const BinConst kc = bin_const<BW>(P, r, c);
+        Yb[S::yidx(c, r)] = filter_bin(P, va[u], cscale(kc.gsv, dc), kc.sX, kc.sY, kc.lut,
+                                       kc.kcn, kb ? kb + r * FBW + c : nullptr, kfirst);
+      }
+    }
+  }
+}
+
+template <int BW, int BH>
+__device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArgs& Wn, int b,
+                                             int lb, c32* Yb, float4* kstate, bool kfirst) {
+  using S = Shape<BW, BH>;
+  constexpr int FBW = S::FBW, NCOL = S::NCOL, H2 = BH / 2;
+  const size_t base = (size_t)b * S::BSTR;
+  const int nwin = Wn.nwin;
+  const float dc = P.dg ? Wn.s[Wn.cur][base].x : 0.0f;
+  float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
+  for (int q = lb; q < H2; q += S::LPB) {
+    // unit 0: self-paired rows 0 and bh/2; unit q: rows q and bh-q
+    const int rA = q, rB = q == 0 ? H2 : BH - q;
+    c32 f[4];  // filtered (rA,0), (rA,h), (rB,0), (rB,h)
+#pragma unroll 1
+    for (int k = 0; k < 4; k++) {
+      const int r = k < 2 ? rA : rB;
+      const int col = (k & 1) ? NCOL : 0;
       c32 v[4];
 #pragma unroll
       for (int t = 0; t < 4; t++) {
-        const c32 wa = va[u][t], wb = vb[u][t];
-        // X[r][0] = (W[r] + conj W[-r]) / 2, X[r][h] = (W[r] - conj W[-r]) / 2i
-        v[t] = col == 0      ? cmake(0.5f * (wa.x + wb.x), 0.5f * (wa.y - wb.y))
-               : col == NCOL ? cmake(0.5f * (wa.y + wb.y), -0.5f * (wa.x - wb.x))
-                             : wa;
+        const c32 wa = t < nwin ? Yb[S::st(t, r)] : cmake(0.0f, 0.0f);
+        const c32 wb = t < nwin ? Yb[S::st(t, q == 0 ? r : BH - r)] : cmake(0.0f, 0.0f);
+        // X[r][0] = (W[r] + conj W[-r]) / 2, X[r][h] = (W[r] - conj W[-r]) / 2i;
+        // for a self-paired row (W[-r] = W[r]) that is (Re W[r], Im W[r])
+        v[t] = col == 0 ? cmake(0.5f * (wa.x + wb.x), 0.5f * (wa.y - wb.y))
+                        : cmake(0.5f * (wa.y + wb.y), -0.5f * (wa.x - wb.x));
       }
       const BinConst kc = bin_const<BW>(P, r, col);
-      Yg[col * S::YP + r] = filter_bin(P, v, cscale(kc.gsv, dc), kc.sX, kc.sY, kc.lut,
-                                       kc.kcn, kb ? kb + r * FBW + col : nullptr, kfirst);
+      const c32 y = filter_bin(P, v, cscale(kc.gsv, dc), kc.sX, kc.sY, kc.lut, kc.kcn,
+                               kb ? kb + r * FBW + col : nullptr, kfirst);
+      // f[k] with a runtime k would live in scratch: select instead
+      f[0] = k == 0 ? y : f[0];
+      f[1] = k == 1 ? y : f[1];
+      f[2] = k == 2 ? y : f[2];
+      f[3] = k == 3 ? y : f[3];
+    }
+    if (q == 0) {
+      // the Hermitian part of a self-paired bin is its real part
+      Yb[S::yidx(0, rA)] = cmake(f[0].x, f[1].x);
+      Yb[S::yidx(0, rB)] = cmake(f[2].x, f[3].x);
+    } else {
+      const c32 h0 = cscale(cadd(f[0], cconj(f[2])), 0.5f);
+      const c32 hh = cscale(cadd(f[1], cconj(f[3])), 0.5f);
+      Yb[S::yidx(0, rA)] = cmake(h0.x - hh.y, h0.y + hh.x);
+      Yb[S::yidx(0, rB)] = cmake(h0.x + hh.y, -h0.y + hh.x);
     }
   }
 }
 
-// Packs the filtered DC and Nyquist columns back into column 0 as
-// W'[r] = H0[r] + i Hh[r], H = (F[r] + conj F[-r]) / 2 the Hermitian part
-// (the only part a real inverse transform keeps). All lanes call.
+// Filter of block b into Yb; contains the barrier between its two passes, so
+// every lane of the workgroup calls it (live = this lane's block exists).
 template <int BW, int BH>
-__device__ __forceinline__ void repack_block(bool live, int lb, c32* Yg) {
-  using S = Shape<BW, BH>;
-  constexpr int NT = (BH + S::LPB - 1) / S::LPB;
-  c32 w[NT];
-#pragma unroll
-  for (int t = 0; t < NT; t++) {
-    const int r = t * S::LPB + lb;
-    w[t] = cmake(0.0f, 0.0f);
-    if (live && r < BH) {
-      const int r2 = (BH - r) & (BH - 1);
-      const c32 f0 = Yg[r], g0 = Yg[r2];
-      const c32 fh = Yg[S::NCOL * S::YP + r], gh = Yg[S::NCOL * S::YP + r2];
-      const c32 h0 = cscale(cadd(f0, cconj(g0)), 0.5f);
-      const c32 hh = cscale(cadd(fh, cconj(gh)), 0.5f);
-      w[t] = cmake(h0.x - hh.y, h0.y + hh.x);
-    }
-  }
+__device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArgs& Wn, int b,
+                                             bool live, int lb, c32* Yb, float4* kstate,
+                                             bool kfirst) {
+  if (live) filter_pass1<BW, BH>(P, Wn, b, lb, Yb, kstate, kfirst);
   __syncthreads();
-#pragma unroll
-  for (int t = 0; t < NT; t++) {
-    const int r = t * S::LPB + lb;
-    if (live && r < BH) Yg[r] = w[t];
-  }
+  if (live) filter_pass2<BW, BH>(P, Wn, b, lb, Yb, kstate, kfirst);
   __syncthreads();
 }
 
-// Inverse 2-D C2R of block columns 0..NCOL-1 held in the LDS column buffer
-// (column 0 packed). On return lanes j < NROW hold real rows 2j (xr) and
-// 2j+1 (xi), unnormalised (x bw*bh). All lanes call.
+// Inverse 2-D C2R of the filtered block held in its LDS area (column 0
+// packed). On return lanes j < NROW hold real rows 2j (xr) and 2j+1 (xi),
+// unnormalised (x bw*bh). All lanes call.
 template <int BW, int BH>
-__device__ __forceinline__ void inverse_block(bool live, int j, c32* Yg, float (&xr)[BW],
+__device__ __forceinline__ void inverse_block(bool live, int j, c32* Yb, float (&xr)[BW],
                                               float (&xi)[BW]) {
   using S = Shape<BW, BH>;
   if (live && j < S::NCOL) {
     float yr[BH], yi[BH];
-    c32* Y = Yg + j * S::YP;
 #pragma unroll
     for (int r = 0; r < BH; r++) {
-      yr[r] = Y[r].x;
-      yi[r] = Y[r].y;
+      const c32 v = Yb[S::yidx(j, r)];
+      yr[r] = v.x;
+      yi[r] = v.y;
     }
     fft_reg<BH, true>(yr, yi);
 #pragma unroll
-    for (int r = 0; r < BH; r++) Y[r] = cmake(yr[r], yi[r]);
+    for (int r = 0; r < BH; r++) Yb[S::yidx(j, r)] = cmake(yr[r], yi[r]);
   }
   __syncthreads();
   if (live && j < S::NROW) {
     const int p = j;
     {
-      const c32 v = Yg[2 * p], u = Yg[2 * p + 1];
+      const c32 v = Yb[S::yidx(0, 2 * p)], u = Yb[S::yidx(0, 2 * p + 1)];
       xr[0] = v.x;
       xi[0] = u.x;
       xr[BW / 2] = v.y;
@@ -456,7 +479,7 @@ __device__ __forceinline__ void inverse_block(bool live, int j, c32* Yg, float (
     }
 #pragma unroll
     for (int c = 1; c < S::NCOL; c++) {
-      const c32 a = Yg[c * S::YP + 2 * p], q = Yg[c * S::YP + 2 * p + 1];
+      const c32 a = Yb[S::yidx(c, 2 * p)], q = Yb[S::yidx(c, 2 * p + 1)];
       xr[c] = a.x - q.y;
       xi[c] = a.y + q.x;
       xr[BW - c] = a.x + q.y;
@@ -476,12 +499,10 @@ __global__ __launch_bounds__(64) void k_filter_inverse(PlaneArgs P, WindowArgs W
   const int lane = threadIdx.x, g = lane / S::LPB, j = lane - g * S::LPB;
   const int b = blockIdx.x * S::G + g;
   const bool live = b < P.nb;
-  c32* Yg = lds + g * S::FBW * S::YP;
-  if (live) filter_block<BW, BH>(P, Wn, b, j, Yg, kstate, kfirst != 0);
-  __syncthreads();
-  repack_block<BW, BH>(live, j, Yg);
+  c32* Yb = lds + g * S::YS;
+  filter_block<BW, BH>(P, Wn, b, live, j, Yb, kstate, kfirst != 0);
   float xr[BW], xi[BW];
-  inverse_block<BW, BH>(live, j, Yg, xr, xi);
+  inverse_block<BW, BH>(live, j, Yb, xr, xi);
   if (live && j < S::NROW) {
     int x0, y0, par;
     block_origin(P, b, x0, y0, par);

@@ -46,7 +46,7 @@ struct SweepShape {
   static constexpr int SP = CW + 1;               // strip pitch in floats
   static constexpr int NT = 64 * WPW;             // threads per workgroup
   static constexpr int NCARRY = (SY * CW + NT - 1) / NT;
-  static constexpr int YFLOATS = 2 * S::G * S::FBW * S::YP * WPW;
+  static constexpr int YFLOATS = 2 * S::G * S::YS * WPW;
   static constexpr int LDS_FLOATS = YFLOATS > BH * SP ? YFLOATS : BH * SP;
 };
 
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   const int nlive = min(NBW, P.ncols - n0);
   const int n = n0 + w * S::G + g;  // this lane's block column
   const bool colLive = n < P.ncols;
-  c32* Yg = reinterpret_cast<c32*>(lds) + (w * S::G + g) * S::FBW * S::YP;
+  c32* Yb = reinterpret_cast<c32*>(lds) + (w * S::G + g) * S::YS;
   float* strp = lds;
   float* part = A.part + (size_t)((band & 1) * 2 + (strip & 1)) * P.W * P.H;
   const int validCols = (nlive + 1) * SX;
@@ -75,12 +75,10 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
 
   for (int m = m0; m < mEnd; m++) {
     const int b = m * P.ncols + n;
-    __syncthreads();  // the previous row's strip reads are done: LDS is the Y buffers again
-    if (colLive) filter_block<BW, BH>(P, Wn, b, j, Yg, kstate, kfirst != 0);
-    __syncthreads();
-    repack_block<BW, BH>(colLive, j, Yg);
+    __syncthreads();  // the previous row's strip reads are done: LDS is the block areas again
+    filter_block<BW, BH>(P, Wn, b, colLive, j, Yb, kstate, kfirst != 0);
     float xr[BW], xi[BW];
-    inverse_block<BW, BH>(colLive, j, Yg, xr, xi);
+    inverse_block<BW, BH>(colLive, j, Yb, xr, xi);
     const bool rowLane = colLive && j < S::NROW;
     if (rowLane) {
       // half overlap: both parities use the same separable synthesis window
