@@ -115,12 +115,61 @@ __device__ __forceinline__ void block_origin(const PlaneArgs& P, int b, int& x0,
   }
 }
 
-// Sample -> float (src/cpu/reference.cpp:82-105; src/cl/kernels.h:22-34).
-__device__ __forceinline__ float fetch(const PlaneArgs& P, const void* plane, int x, int y) {
-  const size_t i = (size_t)mirror(y, P.H) * P.W + mirror(x, P.W);
-  if (P.pix == 8) return (float)static_cast<const uint8_t*>(plane)[i] * (1.0f / 255.0f);
-  if (P.pix == 16) return (float)static_cast<const uint16_t*>(plane)[i] * P.inNorm;
-  return static_cast<const float*>(plane)[i];
+// Sample -> float (src/cpu/reference.cpp:82-105; src/cl/kernels.h:22-34):
+// Samples x0..x0+BW-1 of plane row y (already mirrored) as floats, minus
+// chromaNorm. Rows that lie inside the plane at a 16-byte aligned address are
+// read with 16-byte loads; other rows go sample by sample through the mirror.
+template <int BW>
+__device__ __forceinline__ void load_row(const PlaneArgs& P, const void* plane, int x0, int y,
+                                         float (&z)[BW]) {
+  const int bps = P.pix == 8 ? 1 : (P.pix == 16 ? 2 : 4);
+  const char* row = static_cast<const char*>(plane) + ((size_t)y * P.W + x0) * bps;
+  const bool vec = (BW * bps) % 16 == 0 && x0 >= 0 && x0 + BW <= P.W &&
+                   (reinterpret_cast<uintptr_t>(row) & 15) == 0;
+  if (vec) {
+    const uint4* q = reinterpret_cast<const uint4*>(row);
+    if (P.pix == 8) {
+#pragma unroll
+      for (int i = 0; i < BW / 16; i++) {
+        const uint4 v = q[i];
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+          z[16 * i + k] = (float)((w[k >> 2] >> (8 * (k & 3))) & 0xffu) * (1.0f / 255.0f);
+      }
+    } else if (P.pix == 16) {
+#pragma unroll
+      for (int i = 0; i < BW / 8; i++) {
+        const uint4 v = q[i];
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+          z[8 * i + k] = (float)((w[k >> 1] >> (16 * (k & 1))) & 0xffffu) * P.inNorm;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < BW / 4; i++) {
+        const float4 v = reinterpret_cast<const float4*>(row)[i];
+        z[4 * i] = v.x;
+        z[4 * i + 1] = v.y;
+        z[4 * i + 2] = v.z;
+        z[4 * i + 3] = v.w;
+      }
+    }
+  } else {
+    const size_t base = (size_t)y * P.W;
+#pragma unroll
+    for (int k = 0; k < BW; k++) {
+      const size_t i = base + mirror(x0 + k, P.W);
+      float v;
+      if (P.pix == 8) v = (float)static_cast<const uint8_t*>(plane)[i] * (1.0f / 255.0f);
+      else if (P.pix == 16) v = (float)static_cast<const uint16_t*>(plane)[i] * P.inNorm;
+      else v = static_cast<const float*>(plane)[i];
+      z[k] = v;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < BW; k++) z[k] -= P.chromaNorm;
 }
 
 // --------------------------------------------------------------- analysis
@@ -137,11 +186,8 @@ __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* pl
     block_origin(P, b, x0, y0, par);
     float zr[BW], zi[BW];
     const int r0 = 2 * j;
-#pragma unroll
-    for (int k = 0; k < BW; k++) {
-      zr[k] = fetch(P, plane, x0 + k, y0 + r0) - P.chromaNorm;
-      zi[k] = fetch(P, plane, x0 + k, y0 + r0 + 1) - P.chromaNorm;
-    }
+    load_row<BW>(P, plane, x0, mirror(y0 + r0, P.H), zr);
+    load_row<BW>(P, plane, x0, mirror(y0 + r0 + 1, P.H), zi);
     if (P.mode == 1) {
       // separable window: a uniform column factor times a per-lane row factor
       const float fy0 = P.anaY[r0], fy1 = P.anaY[r0 + 1];

@@ -33,6 +33,9 @@ struct float4 {
   float x, y, z, w;
 };
 inline float4 make_float4(float x, float y, float z, float w) { return float4{x, y, z, w}; }
+struct uint4 {
+  unsigned x, y, z, w;
+};
 struct int2 {
   int x, y;
 };
