@@ -47,4 +47,18 @@ __device__ __forceinline__ float fsqrt(float x) {
 #endif
 }
 
+// Orders LDS accesses among the lanes of one wavefront. The LDS instructions
+// of a wave execute in issue order, so a compiler fence at wavefront scope
+// around the wave barrier suffices for data that stays inside an LDS area
+// owned by the calling wave; areas shared between waves need __syncthreads.
+__device__ __forceinline__ void wave_sync() {
+#if defined(FFT3D_CPU_EMU)
+  emu_wave_sync();
+#else
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
+}
+
 } // namespace fk

@@ -175,7 +175,8 @@ __device__ __forceinline__ void load_row(const PlaneArgs& P, const void* plane, 
 // --------------------------------------------------------------- analysis
 
 // Windowed 2-D R2C FFT of block b. On return, lanes j < NCOL hold packed
-// column j of the spectrum in (yr[r], yi[r]), r < BH. All lanes must call.
+// column j of the spectrum in (yr[r], yi[r]), r < BH. All lanes of the wave
+// call it; its LDS area belongs to the wave.
 template <int BW, int BH>
 __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* plane, int b,
                                               bool live, int g, int j, c32* lds,
@@ -210,7 +211,7 @@ __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* pl
 #pragma unroll
     for (int k = 0; k < BW; k++) Z[k] = cmake(zr[k], zi[k]);
   }
-  __syncthreads();
+  wave_sync();
   if (live && j < S::NCOL) {
     const int c = j;
     const c32* Zg = lds + g * S::NROW * S::ZP;
@@ -482,21 +483,23 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
   }
 }
 
-// Filter of block b into Yb; contains the barrier between its two passes, so
-// every lane of the workgroup calls it (live = this lane's block exists).
+// Filter of block b into Yb; contains the wave barrier between its two passes,
+// so every lane of the wave calls it (live = this lane's block exists). Yb must
+// belong to the calling wave.
 template <int BW, int BH>
 __device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              bool live, int lb, c32* Yb, float4* kstate,
                                              bool kfirst) {
   if (live) filter_pass1<BW, BH>(P, Wn, b, lb, Yb, kstate, kfirst);
-  __syncthreads();
+  wave_sync();
   if (live) filter_pass2<BW, BH>(P, Wn, b, lb, Yb, kstate, kfirst);
-  __syncthreads();
+  wave_sync();
 }
 
 // Inverse 2-D C2R of the filtered block held in its LDS area (column 0
 // packed). On return lanes j < NROW hold real rows 2j (xr) and 2j+1 (xi),
-// unnormalised (x bw*bh). All lanes call.
+// unnormalised (x bw*bh). All lanes of the wave call it; Yb belongs to the
+// wave.
 template <int BW, int BH>
 __device__ __forceinline__ void inverse_block(bool live, int j, c32* Yb, float (&xr)[BW],
                                               float (&xi)[BW]) {
@@ -513,7 +516,7 @@ __device__ __forceinline__ void inverse_block(bool live, int j, c32* Yb, float (
 #pragma unroll
     for (int r = 0; r < BH; r++) Yb[S::yidx(j, r)] = cmake(yr[r], yi[r]);
   }
-  __syncthreads();
+  wave_sync();
   if (live && j < S::NROW) {
     const int p = j;
     {

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <barrier>
+#include <memory>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -46,8 +47,12 @@ using std::min;
 inline thread_local dim3 threadIdx, blockIdx;
 inline dim3 blockDim, gridDim;
 inline std::barrier<>* emu_barrier_ = nullptr;
+inline thread_local std::barrier<>* emu_wave_barrier_ = nullptr;
 
 inline void __syncthreads() { emu_barrier_->arrive_and_wait(); }
+// Barrier over the 64 lanes of the calling lane's wavefront only, so that a
+// kernel relying on it for data shared across waves races here as on a GPU.
+inline void emu_wave_sync() { emu_wave_barrier_->arrive_and_wait(); }
 inline float emu_sqrtf(float x) { return std::sqrt(x); }
 using std::copysign;
 using std::floor;
@@ -62,11 +67,15 @@ inline void emu_launch(dim3 grid, dim3 block, const std::function<void()>& fn) {
   gridDim = grid;
   std::barrier<> bar(nt);
   emu_barrier_ = &bar;
+  std::vector<std::unique_ptr<std::barrier<>>> waves;
+  for (unsigned w = 0; w * 64 < nt; w++)
+    waves.emplace_back(new std::barrier<>(std::min<unsigned>(64, nt - w * 64)));
   std::vector<std::thread> th;
   th.reserve(nt);
   for (unsigned t = 0; t < nt; t++) {
     th.emplace_back([&, t] {
       threadIdx = dim3(t % block.x, (t / block.x) % block.y, t / (block.x * block.y));
+      emu_wave_barrier_ = waves[t / 64].get();
       for (unsigned bz = 0; bz < grid.z; bz++)
         for (unsigned by = 0; by < grid.y; by++)
           for (unsigned bx = 0; bx < grid.x; bx++) {
