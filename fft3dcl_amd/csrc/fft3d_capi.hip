@@ -82,6 +82,33 @@ void set_err(char* err, size_t n, const std::string& s) {
   }
 }
 
+// True when the separable factors reproduce the sharpen and degrid tables to
+// float rounding (the fused sweep builds the per-bin constants from them). A
+// zero scutoff gives NaN entries that no product of factors reproduces.
+bool sepMatches(const fft3d::Tables& t, const fft3d::Params& p) {
+  const int fbw = p.bw / 2 + 1;
+  if (t.sepRow.size() != 4u * p.bh || t.sepCol.size() != 4u * fbw) return false;
+  for (int r = 0; r < p.bh; r++)
+    for (int c = 0; c < fbw; c++) {
+      const float* R = &t.sepRow[4 * r];
+      const float* C = &t.sepCol[4 * c];
+      const int i = r * fbw + c;
+      if (p.sharpenOn) {
+        const double v = (double)R[3] - (double)R[0] * C[0], want = t.sharpen[i];
+        if (!(std::fabs(v - want) <= 1e-6 * (1.0 + std::fabs(want)))) return false;
+      }
+      if (p.degridOn) {
+        const double re = (double)C[1] * R[1] - (double)C[2] * R[2];
+        const double im = (double)C[1] * R[2] + (double)C[2] * R[1];
+        const double scale = 1e-6 * (1.0 + std::fabs((double)p.degrid));
+        if (!(std::fabs(re - t.degrid[2 * i]) <= scale) ||
+            !(std::fabs(im - t.degrid[2 * i + 1]) <= scale))
+          return false;
+      }
+    }
+  return true;
+}
+
 } // namespace
 
 struct fft3d_plane {
@@ -379,7 +406,10 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   // one device table block: 3 windows, pattern, sharpen, degrid, separable factors
   const size_t nwin = (size_t)bw * bh, nbin = (size_t)pl->fbw * bh;
   const size_t sep = 3 * nwin + 4 * nbin;
-  std::vector<float> host(sep + 2 * bw + 2 * bh, 0.0f);
+  const size_t fac = (sep + 2 * bw + 2 * bh + 3) / 4 * 4;  // float4 aligned
+  std::vector<float> host(fac + 4 * (size_t)bh + 4 * (size_t)pl->fbw, 0.0f);
+  std::copy(pl->t.sepRow.begin(), pl->t.sepRow.end(), host.begin() + fac);
+  std::copy(pl->t.sepCol.begin(), pl->t.sepCol.end(), host.begin() + fac + 4 * bh);
   std::copy(pl->t.anaX.begin(), pl->t.anaX.end(), host.begin() + sep);
   std::copy(pl->t.anaY.begin(), pl->t.anaY.end(), host.begin() + sep + bw);
   std::copy(pl->t.synX.begin(), pl->t.synX.end(), host.begin() + sep + bw + bh);
@@ -397,7 +427,7 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   ok = ok && pl->alloc(reinterpret_cast<void**>(&pl->dBlocks), sizeof(float) * (size_t)pl->nb * bw * bh);
   if (ok && p.bm == 0)
     ok = pl->alloc(reinterpret_cast<void**>(&pl->dKalman), sizeof(float4) * (size_t)pl->nb * nbin);
-  pl->sweepOk = p.mode == 1 && p.ow * 2 == bw && p.oh * 2 == bh;
+  pl->sweepOk = p.mode == 1 && p.ow * 2 == bw && p.oh * 2 == bh && sepMatches(pl->t, p);
   if (ok && pl->sweepOk) {
     ok = pl->alloc(reinterpret_cast<void**>(&pl->dPart), sizeof(float) * 4 * (size_t)W * H);
     ok = ok && pl->layoutSweep();
@@ -458,6 +488,8 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   a.anaY = pl->dTables + sep + bw;
   a.synX = pl->dTables + sep + bw + bh;
   a.synY = pl->dTables + sep + 2 * bw + bh;
+  a.sepRow = reinterpret_cast<const float4*>(pl->dTables + fac);
+  a.sepCol = reinterpret_cast<const float4*>(pl->dTables + fac + 4 * bh);
   *out = pl;
   return FFT3D_OK;
 }
@@ -674,6 +706,8 @@ int fft3d_plane_get_table(const fft3d_plane* p, int which, float* dst) {
   case 3: v = &p->t.pattern; break;
   case 4: v = &p->t.sharpen; break;
   case 5: v = &p->t.degrid; break;
+  case 6: v = &p->t.sepRow; break;
+  case 7: v = &p->t.sepCol; break;
   default: return FFT3D_EINVAL;
   }
   if (dst && !v->empty()) std::memcpy(dst, v->data(), v->size() * sizeof(float));

@@ -140,6 +140,12 @@ struct Tables {
   // overlap, winSynA = winSynB = synX[lx]*synY[ly]/(bw*bh) (the inverse
   // transform's normalisation folded into synY). Empty in mode 0.
   std::vector<float> anaX, anaY, synX, synY;
+  // Mode 1 separable factors of the sharpen and degrid tables, 4 floats per
+  // entry. Row r: {shY, gyRe, gyIm, strength}; column c < fbw: {shX, gxRe,
+  // gxIm, 0}. sharpen[r][c] = strength - shY[r] * shX[c] (the exponential of a
+  // sum of a row and a column term) and gs[r][c] = gx[c] * gy[r] (the DFT of a
+  // separable window is the product of the 1-D DFTs), both to float rounding.
+  std::vector<float> sepRow, sepCol;
 };
 
 namespace detail {
@@ -251,6 +257,46 @@ inline Tables buildTables(const Params& p) {
         float d2 = d2v + (float)(i * i) / ((bw / 2.0f) * (bw / 2.0f));
         t.sharpen[j * fbw + i] = p.sharpen * (1.0f - std::exp(-d2 / (2.0f * p.scutoff * p.scutoff)));
       }
+    }
+  }
+  if (p.mode == 1) {
+    t.sepRow.assign(4 * bh, 0.0f);
+    t.sepCol.assign(4 * fbw, 0.0f);
+    std::vector<double> anx, syx, any, syy;
+    ramps(p.ow, p.wintype, anx, syx);
+    ramps(p.oh, p.wintype, any, syy);
+    const std::vector<double> wx = profile(bw, p.ow, anx, false);
+    const std::vector<double> wy = profile(bh, p.oh, any, false);
+    const double sc2 = 2.0 * (double)p.scutoff * p.scutoff;
+    double gx0 = 0.0, gy0 = 0.0;
+    for (double v : wx) gx0 += v;
+    for (double v : wy) gy0 += v;
+    for (int r = 0; r < bh; r++) {
+      const int dj = r < bh / 2 ? r : bh - r;
+      const double d2v = (double)dj * dj * ((double)p.svr * p.svr) / ((bh / 2.0) * (bh / 2.0));
+      double gr = 0.0, gi = 0.0;
+      for (int y = 0; y < bh; y++) {
+        const double ang = -2.0 * kPi * ((long)r * y % bh) / bh;
+        gr += wy[y] * std::cos(ang);
+        gi += wy[y] * std::sin(ang);
+      }
+      const double g = (double)p.degrid / (gx0 * gy0);
+      t.sepRow[4 * r] = p.sharpenOn ? (float)(p.sharpen * std::exp(-d2v / sc2)) : 0.0f;
+      t.sepRow[4 * r + 1] = (float)(gr * g);
+      t.sepRow[4 * r + 2] = (float)(gi * g);
+      t.sepRow[4 * r + 3] = p.sharpen;
+    }
+    for (int c = 0; c < fbw; c++) {
+      const double d2 = (double)c * c / ((bw / 2.0) * (bw / 2.0));
+      double gr = 0.0, gi = 0.0;
+      for (int x = 0; x < bw; x++) {
+        const double ang = -2.0 * kPi * ((long)c * x % bw) / bw;
+        gr += wx[x] * std::cos(ang);
+        gi += wx[x] * std::sin(ang);
+      }
+      t.sepCol[4 * c] = p.sharpenOn ? (float)std::exp(-d2 / sc2) : 0.0f;
+      t.sepCol[4 * c + 1] = (float)gr;
+      t.sepCol[4 * c + 2] = (float)gi;
     }
   }
   if (p.degridOn) {

@@ -55,6 +55,10 @@ struct PlaneArgs {
   const float* anaY;   // bh
   const float* synX;   // bw (half overlap: both block parities share it)
   const float* synY;   // bh
+  // separable factors of the sharpen and degrid tables (mode 1): row r
+  // {shY, gyRe, gyIm, strength}, column c {shX, gxRe, gxIm, 0}
+  const float4* sepRow;  // bh
+  const float4* sepCol;  // bw/2+1
 };
 
 // The temporal window handed to the filter: frames in temporal order.
@@ -353,16 +357,13 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
   return f;
 }
 
-// Per-bin constants of (row r, column col < bw/2+1).
+// Per-bin constants of (row r, column col < bw/2+1): the Wiener and Kalman
+// sigma terms, the sharpen gain and the degrid term d = gs * dc of the block.
 struct BinConst {
   float sX, sY, lut, kcn;
-  c32 gsv;
+  c32 d;
 };
-template <int BW>
-__device__ __forceinline__ BinConst bin_const(const PlaneArgs& P, int r, int col) {
-  constexpr int FBW = BW / 2 + 1;
-  BinConst k;
-  const int i = r * FBW + col;
+__device__ __forceinline__ void bin_sigma(const PlaneArgs& P, int i, BinConst& k) {
   if (P.pat) {
     const float pv = P.pattern[i];
     k.sX = pv * P.beta;
@@ -373,9 +374,36 @@ __device__ __forceinline__ BinConst bin_const(const PlaneArgs& P, int r, int col
     k.sY = P.sigmaY;
     k.kcn = P.kalmanCN;
   }
+}
+// From the full sharpen and degrid tables in device memory.
+template <int BW>
+__device__ __forceinline__ BinConst bin_const(const PlaneArgs& P, int r, int col, float dc) {
+  constexpr int FBW = BW / 2 + 1;
+  BinConst k;
+  const int i = r * FBW + col;
+  bin_sigma(P, i, k);
   k.lut = P.shp ? P.sharpen[i] : 0.0f;
-  k.gsv = P.dg ? P.gs[i] : cmake(0.0f, 0.0f);
+  k.d = P.dg ? cscale(P.gs[i], dc) : cmake(0.0f, 0.0f);
   return k;
+}
+// From the separable factors: rf = row factor {shY, gyRe, gyIm, strength},
+// cf = column factor with its degrid part already scaled by the block DC.
+template <int BW>
+__device__ __forceinline__ BinConst bin_const_sep(const PlaneArgs& P, int r, int col, float4 rf,
+                                                  float4 cf) {
+  constexpr int FBW = BW / 2 + 1;
+  BinConst k;
+  bin_sigma(P, r * FBW + col, k);
+  k.lut = P.shp ? rf.w - rf.x * cf.x : 0.0f;
+  k.d = P.dg ? cmake(cf.y * rf.y - cf.z * rf.z, cf.y * rf.z + cf.z * rf.y) : cmake(0.0f, 0.0f);
+  return k;
+}
+// Column factor of column col scaled for a block of DC dc.
+__device__ __forceinline__ float4 col_factor(const float4* tab, int bh, int col, float dc) {
+  float4 cf = tab[bh + col];
+  cf.y *= dc;
+  cf.z *= dc;
+  return cf;
 }
 
 // Temporal filter of block b into its LDS area Yb (Shape::YS complex).
@@ -384,17 +412,18 @@ __device__ __forceinline__ BinConst bin_const(const PlaneArgs& P, int r, int col
 // filter bin (r, c) into the swizzled column buffer; the lane holding column
 // 0 (the packed DC/Nyquist pair W[r]) stages W[r] of every window frame in
 // LDS instead. All lanes of the block read row r of every frame together, so
-// each 128-byte row line is fetched once. Every lane issues the loads of U
-// rows before any arithmetic.
+// each 128-byte row line is fetched once. Loads go out U rows at a time, one
+// batch ahead of the arithmetic.
 // Pass 2 (after a barrier) filters the DC and Nyquist bins from the staged
 // column: unit q pairs rows q and bh-q (unit 0 bundles the self-paired rows 0
 // and bh/2), i.e. four bins per unit and bh/2 units per block, and writes the
 // repacked W'[r] = H0[r] + i Hh[r], H = (F[r] + conj F[-r]) / 2 the Hermitian
 // part that the real inverse transform keeps.
 // Both passes are called by every lane of the block (lb = lane in block).
-template <int BW, int BH>
+template <int BW, int BH, bool SEP>
 __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArgs& Wn, int b,
-                                             int lb, c32* Yb, float4* kstate, bool kfirst) {
+                                             int lb, c32* Yb, const float4* tab, float4* kstate,
+                                             bool kfirst) {
   using S = Shape<BW, BH>;
   constexpr int FBW = S::FBW, NCOL = S::NCOL;
   constexpr int RS = S::LPB / NCOL;  // lanes sharing a column split its rows
@@ -405,9 +434,10 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   const int c = lb % NCOL, r0 = lb / NCOL;
   const float dc = P.dg ? Wn.s[Wn.cur][base].x : 0.0f;  // block DC: X[0][0] = Re W[0]
   float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
-#pragma unroll 1
-  for (int i0 = 0; i0 < NIT; i0 += U) {
-    c32 va[U][4];
+  float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if constexpr (SEP) cf = col_factor(tab, BH, c, dc);
+  // loads of batch k+1 are in flight while batch k is filtered
+  auto load = [&](int i0, c32(&va)[U][4]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int r = r0 + (i0 + u) * RS;
@@ -417,6 +447,8 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
         if (t < nwin && i0 + u < NIT) va[u][t] = Wn.s[t][base + r * NCOL + c];
       }
     }
+  };
+  auto work = [&](int i0, const c32(&va)[U][4]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (i0 + u >= NIT) continue;
@@ -426,23 +458,43 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
         for (int t = 0; t < 4; t++)
           if (t < nwin) Yb[S::st(t, r)] = va[u][t];
       } else {
-        const BinConst kc = bin_const<BW>(P, r, c);
-        Yb[S::yidx(c, r)] = filter_bin(P, va[u], cscale(kc.gsv, dc), kc.sX, kc.sY, kc.lut,
-                                       kc.kcn, kb ? kb + r * FBW + c : nullptr, kfirst);
+        BinConst kc;
+        if constexpr (SEP) kc = bin_const_sep<BW>(P, r, c, tab[r], cf);
+        else kc = bin_const<BW>(P, r, c, dc);
+        Yb[S::yidx(c, r)] = filter_bin(P, va[u], kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
+                                       kb ? kb + r * FBW + c : nullptr, kfirst);
       }
+    }
+  };
+  constexpr int NB = (NIT + U - 1) / U;
+  c32 va[U][4], vb[U][4];
+  load(0, va);
+#pragma unroll 1
+  for (int k = 0; k < NB; k += 2) {
+    if (k + 1 < NB) load((k + 1) * U, vb);
+    work(k * U, va);
+    if (k + 1 < NB) {
+      if (k + 2 < NB) load((k + 2) * U, va);
+      work((k + 1) * U, vb);
     }
   }
 }
 
-template <int BW, int BH>
+template <int BW, int BH, bool SEP>
 __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArgs& Wn, int b,
-                                             int lb, c32* Yb, float4* kstate, bool kfirst) {
+                                             int lb, c32* Yb, const float4* tab, float4* kstate,
+                                             bool kfirst) {
   using S = Shape<BW, BH>;
   constexpr int FBW = S::FBW, NCOL = S::NCOL, H2 = BH / 2;
   const size_t base = (size_t)b * S::BSTR;
   const int nwin = Wn.nwin;
   const float dc = P.dg ? Wn.s[Wn.cur][base].x : 0.0f;
   float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
+  float4 cf0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cfh = cf0;
+  if constexpr (SEP) {
+    cf0 = col_factor(tab, BH, 0, dc);
+    cfh = col_factor(tab, BH, NCOL, dc);
+  }
   for (int q = lb; q < H2; q += S::LPB) {
     // unit 0: self-paired rows 0 and bh/2; unit q: rows q and bh-q
     const int rA = q, rB = q == 0 ? H2 : BH - q;
@@ -461,8 +513,10 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
         v[t] = col == 0 ? cmake(0.5f * (wa.x + wb.x), 0.5f * (wa.y - wb.y))
                         : cmake(0.5f * (wa.y + wb.y), -0.5f * (wa.x - wb.x));
       }
-      const BinConst kc = bin_const<BW>(P, r, col);
-      const c32 y = filter_bin(P, v, cscale(kc.gsv, dc), kc.sX, kc.sY, kc.lut, kc.kcn,
+      BinConst kc;
+      if constexpr (SEP) kc = bin_const_sep<BW>(P, r, col, tab[r], col == 0 ? cf0 : cfh);
+      else kc = bin_const<BW>(P, r, col, dc);
+      const c32 y = filter_bin(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                kb ? kb + r * FBW + col : nullptr, kfirst);
       // f[k] with a runtime k would live in scratch: select instead
       f[0] = k == 0 ? y : f[0];
@@ -485,14 +539,16 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
 
 // Filter of block b into Yb; contains the wave barrier between its two passes,
 // so every lane of the wave calls it (live = this lane's block exists). Yb must
-// belong to the calling wave.
-template <int BW, int BH>
+// belong to the calling wave. With SEP the sharpen and degrid constants come
+// from the separable factors in tab (LDS: bh row factors, then bw/2+1 column
+// factors); otherwise from the full tables.
+template <int BW, int BH, bool SEP>
 __device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArgs& Wn, int b,
-                                             bool live, int lb, c32* Yb, float4* kstate,
-                                             bool kfirst) {
-  if (live) filter_pass1<BW, BH>(P, Wn, b, lb, Yb, kstate, kfirst);
+                                             bool live, int lb, c32* Yb, const float4* tab,
+                                             float4* kstate, bool kfirst) {
+  if (live) filter_pass1<BW, BH, SEP>(P, Wn, b, lb, Yb, tab, kstate, kfirst);
   wave_sync();
-  if (live) filter_pass2<BW, BH>(P, Wn, b, lb, Yb, kstate, kfirst);
+  if (live) filter_pass2<BW, BH, SEP>(P, Wn, b, lb, Yb, tab, kstate, kfirst);
   wave_sync();
 }
 
@@ -549,7 +605,7 @@ __global__ __launch_bounds__(64) void k_filter_inverse(PlaneArgs P, WindowArgs W
   const int b = blockIdx.x * S::G + g;
   const bool live = b < P.nb;
   c32* Yb = lds + g * S::YS;
-  filter_block<BW, BH>(P, Wn, b, live, j, Yb, kstate, kfirst != 0);
+  filter_block<BW, BH, false>(P, Wn, b, live, j, Yb, nullptr, kstate, kfirst != 0);
   float xr[BW], xi[BW];
   inverse_block<BW, BH>(live, j, Yb, xr, xi);
   if (live && j < S::NROW) {

@@ -47,7 +47,13 @@ struct SweepShape {
   static constexpr int NT = 64 * WPW;             // threads per workgroup
   static constexpr int NCARRY = (SY * CW + NT - 1) / NT;
   static constexpr int YFLOATS = 2 * S::G * S::YS * WPW;
-  static constexpr int LDS_FLOATS = YFLOATS > BH * SP ? YFLOATS : BH * SP;
+  static constexpr int AREA = ((YFLOATS > BH * SP ? YFLOATS : BH * SP) + 3) & ~3;
+  // constant tables kept in LDS after the block/strip area: the separable
+  // sharpen/degrid factors (bh rows, then bw/2+1 columns, float4 each), then
+  // the synthesis window factors synX (bw) and synY (bh)
+  static constexpr int NFAC = 4 * (BH + BW / 2 + 1);
+  static constexpr int TAB = NFAC + BW + BH;
+  static constexpr int LDS_FLOATS = AREA + TAB;
 };
 
 template <int BW, int BH, int WPW>
@@ -57,7 +63,7 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   using S = Shape<BW, BH>;
   using T = SweepShape<BW, BH, WPW>;
   constexpr int SX = T::SX, SY = T::SY, NBW = T::NBW, CW = T::CW, SP = T::SP;
-  __shared__ float lds[T::LDS_FLOATS];
+  __shared__ __attribute__((aligned(16))) float lds[T::LDS_FLOATS];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int g = lane / S::LPB, j = lane - g * S::LPB;
   const int strip = blockIdx.x, band = blockIdx.y;
@@ -72,20 +78,33 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   const int validCols = (nlive + 1) * SX;
   const int x0 = (n0 - 1) * SX;  // image column of strip column 0
   float carry[T::NCARRY];
+  // constant tables (visible to every wave after the row loop's first barrier)
+  float* tabf = lds + T::AREA;
+  for (int i = tid; i < T::TAB; i += T::NT) {
+    float v;
+    if (i < 4 * BH) v = reinterpret_cast<const float*>(P.sepRow)[i];
+    else if (i < T::NFAC) v = reinterpret_cast<const float*>(P.sepCol)[i - 4 * BH];
+    else if (i < T::NFAC + BW) v = P.synX[i - T::NFAC];
+    else v = P.synY[i - T::NFAC - BW];
+    tabf[i] = v;
+  }
+  const float4* tab = reinterpret_cast<const float4*>(tabf);
+  const float* synX = tabf + T::NFAC;
+  const float* synY = synX + BW;
 
   for (int m = m0; m < mEnd; m++) {
     const int b = m * P.ncols + n;
     __syncthreads();  // the previous row's strip reads are done: LDS is the block areas again
-    filter_block<BW, BH>(P, Wn, b, colLive, j, Yb, kstate, kfirst != 0);
+    filter_block<BW, BH, true>(P, Wn, b, colLive, j, Yb, tab, kstate, kfirst != 0);
     float xr[BW], xi[BW];
     inverse_block<BW, BH>(colLive, j, Yb, xr, xi);
     const bool rowLane = colLive && j < S::NROW;
     if (rowLane) {
       // half overlap: both parities use the same separable synthesis window
-      const float fy0 = P.synY[2 * j], fy1 = P.synY[2 * j + 1];
+      const float fy0 = synY[2 * j], fy1 = synY[2 * j + 1];
 #pragma unroll
       for (int k = 0; k < BW; k++) {
-        const float fx = P.synX[k];
+        const float fx = synX[k];
         xr[k] *= fx * fy0;
         xi[k] *= fx * fy1;
       }

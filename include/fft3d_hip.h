@@ -153,7 +153,10 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value);
 int fft3d_plane_get_info(const fft3d_plane* p, fft3d_plane_info* info);
 /* Host copy of a constant table: 0 analysis window, 1/2 synthesis windows
  * (even/odd block columns), 3 sigma pattern, 4 sharpen gains, 5 degrid
- * sample (re,im interleaved). Returns the float count (0 when off). */
+ * sample (re,im interleaved); mode 1 only: 6 row factors {shY, gyRe, gyIm,
+ * strength} per row, 7 column factors {shX, gxRe, gxIm, 0} per column
+ * (sharpen = strength - shY*shX, degrid = gx*gy). Returns the float count
+ * (0 when off). */
 int fft3d_plane_get_table(const fft3d_plane* p, int which, float* dst);
 /* Frame fn's ring spectrum unpacked to the reference layout: nb blocks x bh
  * rows x (bw/2+1) columns of (re, im), i.e. PlaneRef::Spectra

@@ -111,6 +111,31 @@ def test_product_tables_match_oracle(oracle, emu_lib, kw):
         assert np.abs(g - t["degrid"].ravel()).max() < 1e-6
 
 
+@pytest.mark.parametrize("kw", [dict(sharpen=0.4), dict(sharpen=0.7, svr=0.5, scutoff=0.2),
+                                dict(sharpen=1.0, bw=16, bh=64, degrid=0.5),
+                                dict(sharpen=0.3, wintype=1, ow=8, oh=12),
+                                dict(sharpen=0.3, wintype=2, bw=8, bh=8)])
+def test_separable_factors_rebuild_oracle_tables(oracle, emu_lib, kw):
+    """The fused sweep builds each bin's sharpen gain as strength - shY[r]*shX[c]
+    and its degrid sample as gx[c]*gy[r] (tables 6 and 7). Both must rebuild
+    the oracle's full tables (pinned to the reference build) to float
+    rounding."""
+    sp = script(**kw)
+    pl = capi.Plane(capi.make_params(**kw), 150, 110, 8, lib=emu_lib)
+    op = OPlane(oracle, sp, 150, 110, 8)
+    op.build_degrid()
+    t = op.tables()
+    rows = pl.table(6).reshape(-1, 4).astype(np.float64)
+    cols = pl.table(7).reshape(-1, 4).astype(np.float64)
+    assert rows.shape[0] == sp.bh and cols.shape[0] == sp.bw // 2 + 1
+    sharpen = rows[:, 3:4] - rows[:, 0:1] * cols[None, :, 0]
+    want = t["sharpen"].reshape(sp.bh, -1)
+    assert np.abs(sharpen - want).max() < 1e-6
+    gs = (cols[None, :, 1] + 1j * cols[None, :, 2]) * (rows[:, 1:2] + 1j * rows[:, 2:3])
+    assert np.abs(gs - t["degrid"].reshape(sp.bh, -1)).max() < 1e-6
+    pl.close()
+
+
 @pytest.mark.parametrize("bw,bh", [(4, 4), (8, 16), (32, 32), (64, 64), (32, 8)])
 def test_emu_spectrum_matches_oracle(oracle, emu_lib, bw, bh):
     planes = make_clip(2, 90, 70, 1, "f32")
