@@ -6,7 +6,9 @@ sharpen=0.4, degrid=1 (mode 1 default), FP32 spectra ring: the configuration
 the 4K10 frames/s metric is quoted on. A step is one output frame of the
 sequential filter (analysis of the newly needed frame into the ring, temporal
 filter over the 4-frame window, inverse FFT, overlap-add, 10-bit pack), with
-the input planes already resident in HBM and the output written to HBM.
+the input planes already resident in HBM and the output written to HBM. In
+steady state one step is one fused launch (the new frame's analysis inside
+the filter/synthesis sweep) plus the seam combine.
 
 Prints one JSON line. Per-kernel device times come from HIP events recorded on
 the launch stream around every kernel during the timed region; `roofline`
@@ -48,6 +50,9 @@ def kernel_bytes(info, bps_in, bps_out):
         # fused filter/inverse/overlap-add: read the window, write the plane
         # (the seam tiles' share of the plane is written by combine_seams)
         "filter_synth_sweep": spec * info.bt + W * H * bps_out,
+        # the same with the new frame's analysis inside: read its plane and
+        # bt-1 stored spectra, write its spectrum and the plane
+        "forward_filter_synth_sweep": spec * info.bt + W * H * (bps_in + bps_out),
         "combine_seams": 0,
         "step": W * H * (bps_in + bps_out) + spec * info.bt,
     }

@@ -125,6 +125,7 @@ struct fft3d_plane {
 
   float* dTables = nullptr;  // windows, pattern, sharpen, degrid
   void* dStage = nullptr;    // host-upload / host-readback staging (W*H*bps)
+  void* dStageIn = nullptr;  // host upload of a fused analysis (W*H*bps, on first use)
   float* dBlocks = nullptr;  // windowed blocks awaiting overlap-add (unfused path)
   float4* dKalman = nullptr; // bm == 0 state
   bool kalmanFirst = true;
@@ -142,7 +143,7 @@ struct fft3d_plane {
   std::vector<fk::c32*> pool;    // retired spectra kept for reuse
 
   // profiling: events around each launch, resolved in get_profile
-  enum { K_ANALYSIS, K_FILTER, K_SYNTH, K_SWEEP, K_COMBINE, K_COUNT };
+  enum { K_ANALYSIS, K_FILTER, K_SYNTH, K_SWEEP, K_COMBINE, K_FSWEEP, K_COUNT };
   struct Rec {
     int kind;
     hipEvent_t a, b;
@@ -189,6 +190,7 @@ struct fft3d_plane {
     for (auto* m : pool) (void)hipFree(m);
     (void)hipFree(dTables);
     (void)hipFree(dStage);
+    (void)hipFree(dStageIn);
     (void)hipFree(dBlocks);
     (void)hipFree(dKalman);
     (void)hipFree(dPart);
@@ -294,6 +296,29 @@ struct fft3d_plane {
     return code;
   }
 
+  // Window spectra of a process call (errors as fft3d_plane_process).
+  int window(const int* win, int nwin, size_t dstBytes, fk::WindowArgs& w) {
+    const int want = p.bm >= 2 ? p.bt : 1;
+    if (nwin != want) return fail(FFT3D_EINVAL, "window length does not match bt");
+    const size_t need = (size_t)d.width * d.height * bps;
+    if (dstBytes < need) return fail(FFT3D_EINVAL, "output buffer too small");
+    w = fk::WindowArgs{};
+    w.nwin = nwin;
+    w.cur = p.bm >= 2 ? p.current : 0;
+    for (int i = 0; i < 4; i++) {
+      auto it = ring.find(win[i < nwin ? i : 0]);
+      if (it == ring.end()) return fail(FFT3D_EMISSING, "missing spectra in window");
+      w.s[i] = it->second;
+    }
+    return FFT3D_OK;
+  }
+
+  // Filter + synthesis of window w into dst. With fwdMask != 0 (sweep only)
+  // the sweep also transforms fwdPlane into fwdSpec, the window slots in the
+  // mask.
+  int run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipStream_t s,
+          const void* fwdPlane, fk::c32* fwdSpec, int fwdMask);
+
   bool launched(const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -303,6 +328,59 @@ struct fft3d_plane {
     return true;
   }
 };
+
+int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipStream_t s,
+                     const void* fwdPlane, fk::c32* fwdSpec, int fwdMask) {
+  const size_t need = (size_t)d.width * d.height * bps;
+  const int kfirst = (p.bm == 0 && kalmanFirst) ? 1 : 0;
+  void* outp = dst_on_device ? dst : dStage;
+  if (sweepOk && sweepOn) {
+    if (!layoutSweep()) return fail(FFT3D_ENOMEM, "GPU alloc failed (seam list)");
+    fk::SweepArgs a = sw;
+    a.out = outp;
+    a.fwdPlane = fwdPlane;
+    a.fwdSpec = fwdSpec;
+    a.fwdMask = fwdMask;
+    SweepFn ksw = ks.sweep;
+    hipEvent_t t0 = prof_begin(s);
+    FK_LAUNCH(ksw, dim3(a.nstrips, a.nbands), dim3(ks.sweepThreads), s, args, w, dKalman, kfirst,
+              a);
+    prof_end(t0, fwdMask ? K_FSWEEP : K_SWEEP, s);
+    if (!launched(fwdMask ? "analysis/filter/synthesis sweep" : "filter/synthesis sweep"))
+      return FFT3D_ELAUNCH;
+    if (p.bm == 0) kalmanFirst = false;
+    if (a.nseams) {
+      CombineFn kc = &fk::k_combine_seams;
+      t0 = prof_begin(s);
+      FK_LAUNCH(kc, dim3(a.nseams), dim3(256), s, args, a, ks.nbw);
+      prof_end(t0, K_COMBINE, s);
+      if (!launched("seam combine")) return FFT3D_ELAUNCH;
+    }
+  } else {
+    if (fwdMask) return fail(FFT3D_EINVAL, "fused analysis needs the sweep path");
+    const int grid = (nb + ks.G - 1) / ks.G;
+    FilterFn kf = ks.filter_inverse;
+    hipEvent_t t0 = prof_begin(s);
+    FK_LAUNCH(kf, dim3(grid), dim3(64), s, args, w, dKalman, kfirst, dBlocks);
+    prof_end(t0, K_FILTER, s);
+    if (!launched("filter/inverse")) return FFT3D_ELAUNCH;
+    if (p.bm == 0) kalmanFirst = false;
+    SynthFn ksy = &fk::k_synthesize;
+    t0 = prof_begin(s);
+    FK_LAUNCH(ksy, dim3((d.width + 15) / 16, (d.height + 15) / 16), dim3(256), s, args, dBlocks,
+              outp);
+    prof_end(t0, K_SYNTH, s);
+    if (!launched("synthesize")) return FFT3D_ELAUNCH;
+  }
+  if (!dst_on_device) {
+    if (hipMemcpyAsync(dst, dStage, need, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(FFT3D_ELAUNCH, "readback failed");
+    }
+  }
+  return FFT3D_OK;
+}
 
 extern "C" {
 
@@ -541,63 +619,54 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
 int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst, size_t bytes,
                         int dst_on_device, void* stream) {
   if (!p || !window || !dst) return FFT3D_EINVAL;
-  const int want = p->p.bm >= 2 ? p->p.bt : 1;
-  if (nwin != want) return p->fail(FFT3D_EINVAL, "window length does not match bt");
-  const size_t need = (size_t)p->d.width * p->d.height * p->bps;
-  if (bytes < need) return p->fail(FFT3D_EINVAL, "output buffer too small");
-  hipStream_t s = p->stream(stream);
   fk::WindowArgs w{};
-  w.nwin = nwin;
-  w.cur = p->p.bm >= 2 ? p->p.current : 0;
-  for (int i = 0; i < 4; i++) {
-    const int fn = window[i < nwin ? i : 0];
-    auto it = p->ring.find(fn);
-    if (it == p->ring.end()) return p->fail(FFT3D_EMISSING, "missing spectra in window");
-    w.s[i] = it->second;
+  int rc = p->window(window, nwin, bytes, w);
+  if (rc != FFT3D_OK) return rc;
+  return p->run(w, dst, dst_on_device, p->stream(stream), nullptr, nullptr, 0);
+}
+
+int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t src_bytes,
+                                int src_on_device, const int* window, int nwin, void* dst,
+                                size_t dst_bytes, int dst_on_device, void* stream) {
+  if (!p || !src || !window || !dst) return FFT3D_EINVAL;
+  int mask = 0;
+  for (int i = 0; i < nwin && i < 4; i++)
+    if (window[i] == fn) mask |= 1 << i;
+  if (!mask || !(p->sweepOk && p->sweepOn)) {
+    const int rc = fft3d_plane_forward(p, fn, src, src_bytes, src_on_device, stream);
+    if (rc != FFT3D_OK) return rc;
+    return fft3d_plane_process(p, window, nwin, dst, dst_bytes, dst_on_device, stream);
   }
-  const int kfirst = (p->p.bm == 0 && p->kalmanFirst) ? 1 : 0;
-  void* outp = dst_on_device ? dst : p->dStage;
-  if (p->sweepOk && p->sweepOn) {
-    if (!p->layoutSweep()) return p->fail(FFT3D_ENOMEM, "GPU alloc failed (seam list)");
-    fk::SweepArgs a = p->sw;
-    a.out = outp;
-    SweepFn ksw = p->ks.sweep;
-    hipEvent_t t0 = p->prof_begin(s);
-    FK_LAUNCH(ksw, dim3(a.nstrips, a.nbands), dim3(p->ks.sweepThreads), s, p->args, w,
-              p->dKalman, kfirst, a);
-    p->prof_end(t0, fft3d_plane::K_SWEEP, s);
-    if (!p->launched("filter/synthesis sweep")) return FFT3D_ELAUNCH;
-    if (p->p.bm == 0) p->kalmanFirst = false;
-    if (a.nseams) {
-      CombineFn kc = &fk::k_combine_seams;
-      t0 = p->prof_begin(s);
-      FK_LAUNCH(kc, dim3(a.nseams), dim3(256), s, p->args, a, p->ks.nbw);
-      p->prof_end(t0, fft3d_plane::K_COMBINE, s);
-      if (!p->launched("seam combine")) return FFT3D_ELAUNCH;
-    }
-  } else {
-  const int grid = (p->nb + p->ks.G - 1) / p->ks.G;
-  FilterFn kf = p->ks.filter_inverse;
-  hipEvent_t t0 = p->prof_begin(s);
-  FK_LAUNCH(kf, dim3(grid), dim3(64), s, p->args, w, p->dKalman, kfirst, p->dBlocks);
-  p->prof_end(t0, fft3d_plane::K_FILTER, s);
-  if (!p->launched("filter/inverse")) return FFT3D_ELAUNCH;
-  if (p->p.bm == 0) p->kalmanFirst = false;
-  SynthFn ksy = &fk::k_synthesize;
-  t0 = p->prof_begin(s);
-  FK_LAUNCH(ksy, dim3((p->d.width + 15) / 16, (p->d.height + 15) / 16), dim3(256), s, p->args,
-            p->dBlocks, outp);
-  p->prof_end(t0, fft3d_plane::K_SYNTH, s);
-  if (!p->launched("synthesize")) return FFT3D_ELAUNCH;
-  }
-  if (!dst_on_device) {
-    if (hipMemcpyAsync(dst, p->dStage, need, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
+  const size_t need = (size_t)p->d.width * p->d.height * p->bps;
+  if (src_bytes < need) return p->fail(FFT3D_EINVAL, "plane buffer too small");
+  if (nwin != (p->p.bm >= 2 ? p->p.bt : 1))
+    return p->fail(FFT3D_EINVAL, "window length does not match bt");
+  if (dst_bytes < need) return p->fail(FFT3D_EINVAL, "output buffer too small");
+  for (int i = 0; i < nwin; i++)
+    if (window[i] != fn && !p->ring.count(window[i]))
+      return p->fail(FFT3D_EMISSING, "missing spectra in window");
+  hipStream_t s = p->stream(stream);
+  const void* plane = src;
+  if (!src_on_device) {
+    if (!p->dStageIn && !p->alloc(&p->dStageIn, need))
+      return p->fail(FFT3D_ENOMEM, "GPU alloc failed (upload staging)");
+    if (hipMemcpyAsync(p->dStageIn, src, need, hipMemcpyHostToDevice, s) != hipSuccess) {
       (void)hipGetLastError();
-      return p->fail(FFT3D_ELAUNCH, "readback failed");
+      return p->fail(FFT3D_ELAUNCH, "plane upload failed");
     }
+    plane = p->dStageIn;
   }
-  return FFT3D_OK;
+  fk::c32* spec = p->spectrum(fn);
+  if (!spec) return p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed");
+  fk::WindowArgs w{};
+  int rc = p->window(window, nwin, dst_bytes, w);
+  if (rc == FFT3D_OK) rc = p->run(w, dst, dst_on_device, s, plane, spec, mask);
+  if (rc != FFT3D_OK) {
+    // the spectrum may be incomplete: drop it, as a failed forward does
+    p->ring.erase(fn);
+    p->retire(spec);
+  }
+  return rc;
 }
 
 void fft3d_plane_prune(fft3d_plane* p, int lo, int hi) {
@@ -643,7 +712,8 @@ int fft3d_plane_set_profiling(fft3d_plane* p, int on) {
 int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max) {
   if (!p) return FFT3D_EINVAL;
   static const char* names[fft3d_plane::K_COUNT] = {"analysis", "filter_inverse", "synthesize",
-                                                     "filter_synth_sweep", "combine_seams"};
+                                                     "filter_synth_sweep", "combine_seams",
+                                                     "forward_filter_synth_sweep"};
   double ms[fft3d_plane::K_COUNT] = {};
   int n[fft3d_plane::K_COUNT] = {};
   for (auto& r : p->recs) {

@@ -39,7 +39,8 @@ struct Inst {
   int device = 2;
   std::vector<int> planes;              // processed plane indices
   std::vector<fft3d_plane*> gpu;        // parallel to planes
-  std::vector<uint8_t> staging;         // for planes whose rows are padded
+  std::vector<uint8_t> staging;         // input plane whose rows are padded
+  std::vector<uint8_t> stagingOut;      // output plane whose rows are padded
   std::mutex mtx;
   ~Inst() {
     for (auto* p : gpu) fft3d_plane_destroy(p);
@@ -108,36 +109,50 @@ const VSFrame* getFrame(int n, int reason, void* instanceData, void** /*frameDat
     const int pw = vsapi->getFrameWidth(src, p), ph = vsapi->getFrameHeight(src, p);
     const size_t rowBytes = (size_t)pw * bps, planeBytes = rowBytes * ph;
     int rc = FFT3D_OK;
-    int window[4];
+    int window[4], missing[4], nmiss = 0;
     const int nwin = d->bm >= 2 ? d->bt : 1;
-    for (int i = 0; i < nwin && rc == FFT3D_OK; i++) {
+    for (int i = 0; i < nwin; i++) {
       const int fn = clampFrame(d->bm >= 2 ? n - d->current + i : n, vi.numFrames);
       window[i] = fn;
-      if (fft3d_plane_has(g, fn)) continue;
-      const VSFrame* fr = vsapi->getFrameFilter(fn, d->clip, ctx);
+      bool seen = false;
+      for (int k = 0; k < nmiss; k++) seen = seen || missing[k] == fn;
+      if (!seen && !fft3d_plane_has(g, fn)) missing[nmiss++] = fn;
+    }
+    // plane p of a source frame as packed rows: the frame itself when its
+    // stride is the row size (src/fft3dcl.cpp:408-415), else a copy
+    auto packed = [&](const VSFrame* fr) -> const uint8_t* {
       const uint8_t* sp = vsapi->getReadPtr(fr, p);
       const ptrdiff_t ss = vsapi->getStride(fr, p);
-      if ((size_t)ss == rowBytes) {
-        rc = fft3d_plane_forward(g, fn, sp, planeBytes, 0, nullptr);
-      } else {
-        d->staging.resize(planeBytes);
-        for (int y = 0; y < ph; y++)
-          std::memcpy(d->staging.data() + y * rowBytes, sp + y * ss, rowBytes);
-        rc = fft3d_plane_forward(g, fn, d->staging.data(), planeBytes, 0, nullptr);
-      }
+      if ((size_t)ss == rowBytes) return sp;
+      d->staging.resize(planeBytes);
+      for (int y = 0; y < ph; y++)
+        std::memcpy(d->staging.data() + y * rowBytes, sp + y * ss, rowBytes);
+      return d->staging.data();
+    };
+    // the missing frames but the last go through forward; the last one is
+    // transformed by the filter pass itself (forward_process)
+    for (int k = 0; k + 1 < nmiss && rc == FFT3D_OK; k++) {
+      const VSFrame* fr = vsapi->getFrameFilter(missing[k], d->clip, ctx);
+      rc = fft3d_plane_forward(g, missing[k], packed(fr), planeBytes, 0, nullptr);
       vsapi->freeFrame(fr);
     }
     if (rc == FFT3D_OK) {
       uint8_t* dp = vsapi->getWritePtr(dst, p);
       const ptrdiff_t ds = vsapi->getStride(dst, p);
-      if ((size_t)ds == rowBytes) {
-        rc = fft3d_plane_process(g, window, nwin, dp, planeBytes, 0, nullptr);
+      const bool direct = (size_t)ds == rowBytes;  // straight into the frame
+      if (!direct) d->stagingOut.resize(planeBytes);
+      uint8_t* out = direct ? dp : d->stagingOut.data();
+      if (nmiss) {
+        const int fn = missing[nmiss - 1];
+        const VSFrame* fr = vsapi->getFrameFilter(fn, d->clip, ctx);
+        rc = fft3d_plane_forward_process(g, fn, packed(fr), planeBytes, 0, window, nwin, out,
+                                         planeBytes, 0, nullptr);
+        vsapi->freeFrame(fr);
       } else {
-        d->staging.resize(planeBytes);
-        rc = fft3d_plane_process(g, window, nwin, d->staging.data(), planeBytes, 0, nullptr);
-        for (int y = 0; rc == FFT3D_OK && y < ph; y++)
-          std::memcpy(dp + y * ds, d->staging.data() + y * rowBytes, rowBytes);
+        rc = fft3d_plane_process(g, window, nwin, out, planeBytes, 0, nullptr);
       }
+      for (int y = 0; rc == FFT3D_OK && !direct && y < ph; y++)
+        std::memcpy(dp + y * ds, out + y * rowBytes, rowBytes);
     }
     if (rc != FFT3D_OK) {
       std::string msg = "FFT3DCL: GPU processing failed at frame " + std::to_string(n) + ": " +

@@ -61,4 +61,17 @@ __device__ __forceinline__ void wave_sync() {
 #endif
 }
 
+// wave_sync that also orders global-memory accesses of the wave's lanes: a
+// workgroup-scope release (the wave's stores complete) before the barrier and
+// the matching acquire after it.
+__device__ __forceinline__ void wave_sync_global() {
+#if defined(FFT3D_CPU_EMU)
+  emu_wave_sync();
+#else
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#endif
+}
+
 } // namespace fk

@@ -178,13 +178,13 @@ __device__ __forceinline__ void load_row(const PlaneArgs& P, const void* plane, 
 
 // --------------------------------------------------------------- analysis
 
-// Windowed 2-D R2C FFT of block b. On return, lanes j < NCOL hold packed
-// column j of the spectrum in (yr[r], yi[r]), r < BH. All lanes of the wave
-// call it; its LDS area belongs to the wave.
-template <int BW, int BH>
+// Windowed 2-D R2C FFT of block b. Lanes j < NCOL end with packed column j
+// of the spectrum and call emit(yr, yi) with it (r < BH). All lanes of the wave
+// call forward_block; its LDS area belongs to the wave. emit may write the
+// LDS area: within a wave the transpose reads precede it in issue order.
+template <int BW, int BH, class Emit>
 __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* plane, int b,
-                                              bool live, int g, int j, c32* lds,
-                                              float (&yr)[BH], float (&yi)[BH]) {
+                                              bool live, int g, int j, c32* lds, Emit&& emit) {
   using S = Shape<BW, BH>;
   if (live && j < S::NROW) {
     int x0, y0, par;
@@ -218,6 +218,7 @@ __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* pl
   wave_sync();
   if (live && j < S::NCOL) {
     const int c = j;
+    float yr[BH], yi[BH];
     const c32* Zg = lds + g * S::NROW * S::ZP;
 #pragma unroll
     for (int p = 0; p < S::NROW; p++) {
@@ -239,6 +240,7 @@ __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* pl
       }
     }
     fft_reg<BH, false>(yr, yi);
+    emit(yr, yi);
   }
 }
 
@@ -249,13 +251,12 @@ __global__ __launch_bounds__(64) void k_analysis(PlaneArgs P, const void* plane,
   const int lane = threadIdx.x, g = lane / S::LPB, j = lane - g * S::LPB;
   const int b = blockIdx.x * S::G + g;
   const bool live = b < P.nb;
-  float yr[BH], yi[BH];
-  forward_block<BW, BH>(P, plane, b, live, g, j, lds, yr, yi);
-  if (live && j < S::NCOL) {
-    c32* d = spec + (size_t)b * S::BSTR + j;
+  c32* d = spec + (size_t)b * S::BSTR + j;
+  forward_block<BW, BH>(P, plane, b, live, g, j, lds,
+                        [&](const float(&yr)[BH], const float(&yi)[BH]) {
 #pragma unroll
-    for (int r = 0; r < BH; r++) d[r * S::NCOL] = cmake(yr[r], yi[r]);
-  }
+                          for (int r = 0; r < BH; r++) d[r * S::NCOL] = cmake(yr[r], yi[r]);
+                        });
 }
 
 // ---------------------------------------------------------------- filter
@@ -423,7 +424,7 @@ __device__ __forceinline__ float4 col_factor(const float4* tab, int bh, int col,
 template <int BW, int BH, bool SEP>
 __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
-                                             bool kfirst) {
+                                             bool kfirst, float dc) {
   using S = Shape<BW, BH>;
   constexpr int FBW = S::FBW, NCOL = S::NCOL;
   constexpr int RS = S::LPB / NCOL;  // lanes sharing a column split its rows
@@ -432,7 +433,6 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   const size_t base = (size_t)b * S::BSTR;
   const int nwin = Wn.nwin;
   const int c = lb % NCOL, r0 = lb / NCOL;
-  const float dc = P.dg ? Wn.s[Wn.cur][base].x : 0.0f;  // block DC: X[0][0] = Re W[0]
   float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
   float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if constexpr (SEP) cf = col_factor(tab, BH, c, dc);
@@ -444,7 +444,8 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
 #pragma unroll
       for (int t = 0; t < 4; t++) {
         va[u][t] = cmake(0.0f, 0.0f);
-        if (t < nwin && i0 + u < NIT) va[u][t] = Wn.s[t][base + r * NCOL + c];
+        if (t < nwin && i0 + u < NIT)
+          va[u][t] = Wn.s[t][base + r * NCOL + c];
       }
     }
   };
@@ -483,12 +484,10 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
 template <int BW, int BH, bool SEP>
 __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
-                                             bool kfirst) {
+                                             bool kfirst, float dc) {
   using S = Shape<BW, BH>;
   constexpr int FBW = S::FBW, NCOL = S::NCOL, H2 = BH / 2;
-  const size_t base = (size_t)b * S::BSTR;
   const int nwin = Wn.nwin;
-  const float dc = P.dg ? Wn.s[Wn.cur][base].x : 0.0f;
   float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
   float4 cf0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cfh = cf0;
   if constexpr (SEP) {
@@ -541,14 +540,22 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
 // so every lane of the wave calls it (live = this lane's block exists). Yb must
 // belong to the calling wave. With SEP the sharpen and degrid constants come
 // from the separable factors in tab (LDS: bh row factors, then bw/2+1 column
-// factors); otherwise from the full tables.
+// factors); otherwise from the full tables. When bit `cur` of fwdMask is set,
+// the current frame was transformed by this wave just before and its block DC
+// is read from the last slot of Yb.
 template <int BW, int BH, bool SEP>
 __device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              bool live, int lb, c32* Yb, const float4* tab,
-                                             float4* kstate, bool kfirst) {
-  if (live) filter_pass1<BW, BH, SEP>(P, Wn, b, lb, Yb, tab, kstate, kfirst);
+                                             float4* kstate, bool kfirst, int fwdMask = 0) {
+  using S = Shape<BW, BH>;
+  // block DC X[0][0] = Re W[0] of the current frame, for the degrid term
+  float dc = 0.0f;
+  if (live && P.dg)
+    dc = ((fwdMask >> Wn.cur) & 1) ? Yb[S::YS - 1].x : Wn.s[Wn.cur][(size_t)b * S::BSTR].x;
+  if (fwdMask) wave_sync();  // pass 1 stages over that slot
+  if (live) filter_pass1<BW, BH, SEP>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
   wave_sync();
-  if (live) filter_pass2<BW, BH, SEP>(P, Wn, b, lb, Yb, tab, kstate, kfirst);
+  if (live) filter_pass2<BW, BH, SEP>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
   wave_sync();
 }
 

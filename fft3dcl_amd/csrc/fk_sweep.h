@@ -9,7 +9,8 @@
 // (m-1..m) x (n-1..n). A workgroup of WPW waves owns a strip of NBW = WPW*G
 // adjacent block columns and a band of R block rows, and sweeps down the
 // band one block row at a time:
-//   1. each wave filters + inverse-transforms its G blocks (fk_kernels.h);
+//   1. each wave filters + inverse-transforms its G blocks (fk_kernels.h),
+//      after transforming them for a frame being analysed in the same pass;
 //   2. the windowed blocks are overlap-added along the row in an LDS strip,
 //      even block columns storing and odd ones adding (fixed order: the
 //      result is bit-identical run to run, no atomics);
@@ -35,6 +36,11 @@ struct SweepArgs {
   void* out;         // packed output plane (u8/u16/f32)
   const int2* seams; // border tiles (tile column, tile row)
   int nseams;
+  // Fused analysis of one new frame (fwdMask != 0): its plane, the ring
+  // spectrum to write, and the window slots that hold it (bit t = slot t).
+  const void* fwdPlane;
+  c32* fwdSpec;
+  int fwdMask;
 };
 
 template <int BW, int BH, int WPW>
@@ -95,7 +101,19 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   for (int m = m0; m < mEnd; m++) {
     const int b = m * P.ncols + n;
     __syncthreads();  // the previous row's strip reads are done: LDS is the block areas again
-    filter_block<BW, BH, true>(P, Wn, b, colLive, j, Yb, tab, kstate, kfirst != 0);
+    if (A.fwdMask) {
+      // analysis of the new frame into the ring; the block DC also goes to
+      // the last slot of Yb, which the forward transpose never touches
+      c32* d = A.fwdSpec + (size_t)b * S::BSTR + j;
+      forward_block<BW, BH>(P, A.fwdPlane, b, colLive, 0, j, Yb,
+                            [&](const float(&yr)[BH], const float(&yi)[BH]) {
+#pragma unroll
+                              for (int r = 0; r < BH; r++) d[r * S::NCOL] = cmake(yr[r], yi[r]);
+                              if (j == 0) Yb[S::YS - 1] = cmake(yr[0], yi[0]);
+                            });
+      wave_sync_global();  // the ring stores are read back by the filter
+    }
+    filter_block<BW, BH, true>(P, Wn, b, colLive, j, Yb, tab, kstate, kfirst != 0, A.fwdMask);
     float xr[BW], xi[BW];
     inverse_block<BW, BH>(colLive, j, Yb, xr, xi);
     const bool rowLane = colLive && j < S::NROW;

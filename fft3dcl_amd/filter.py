@@ -48,8 +48,11 @@ class Instance:
 
     def __init__(self, color_family: int, width: int, height: int, bits: int,
                  sample_type: int, num_frames: int, ssw: int = 1, ssh: int = 1,
-                 lib=None, **kw):
+                 lib=None, fuse: bool = True, **kw):
+        """fuse: transform the last missing window frame inside the filter pass
+        (fft3d_plane_forward_process) instead of with its own forward call."""
         device = int(kw.pop("device", 2))
+        self.fuse = fuse
         if device == 0:
             raise ValueError("FFT3DCL: device=0 selects the CPU backend, which this HIP "
                              "build does not ship (use device=1 or 2)")
@@ -86,10 +89,18 @@ class Instance:
         for k, q in enumerate(self.planes):
             obj = self.objs[k]
             win = self.window(n)
-            for fn in win:
-                if not obj.has(fn):
+            missing = [fn for fn in dict.fromkeys(win) if not obj.has(fn)]
+            if self.fuse and missing:
+                # the last missing frame is transformed inside the filter pass
+                for fn in missing[:-1]:
                     obj.forward(fn, src_planes[q](fn), on_device, stream)
-            obj.process(win, dst_planes[q], on_device, stream)
+                fn = missing[-1]
+                obj.forward_process(fn, src_planes[q](fn), win, dst_planes[q], on_device,
+                                    on_device, stream)
+            else:
+                for fn in missing:
+                    obj.forward(fn, src_planes[q](fn), on_device, stream)
+                obj.process(win, dst_planes[q], on_device, stream)
             if self.bm >= 2:
                 obj.prune(n - self.current - 1, n - self.current + self.bt)
             elif self.bm != 0:

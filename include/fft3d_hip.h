@@ -111,6 +111,19 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes,
 int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst,
                         size_t bytes, int dst_on_device, void* stream);
 
+/* fft3d_plane_forward(fn, src) followed by fft3d_plane_process(window, dst),
+ * as one call: the per-frame sequence of the reference's scheduler, which
+ * forwards each window frame missing from the cache and then processes the
+ * window (src/fft3dcl.cpp:396-442). Same arguments, errors, ring contents and
+ * output as the two calls. When fn is missing, belongs to the window and the
+ * geometry runs the fused sweep, frame fn is transformed inside the sweep
+ * (its spectrum still lands in the ring) instead of by its own launch, which
+ * saves one spectrum write-then-read per frame. Otherwise it runs the two
+ * calls. */
+int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t src_bytes,
+                                int src_on_device, const int* window, int nwin, void* dst,
+                                size_t dst_bytes, int dst_on_device, void* stream);
+
 /* Drops ring entries outside [lo, hi]; a negative bound is open
  * (GpuRef::pruneCache, src/cl/gpu.cpp:507-517). Retired spectra go to a
  * bounded reuse pool. */

@@ -232,3 +232,56 @@ def test_emu_sweep_seams_match_oracle(oracle, emu_lib, kind, kw, W, H, R):
             assert max_abs(out, want[0]) < 1e-5
         else:
             assert lsb_diff(out, want[0]) <= 1
+
+
+@pytest.mark.parametrize("kind,kw,W,H", [
+    ("u8", dict(bt=3), 300, 90),
+    ("u10", dict(bt=4, sharpen=0.4), 270, 70),
+    ("f32", dict(bt=0, kratio=1.5), 200, 60),
+    ("u8", dict(bt=1, bw=16, bh=64), 300, 140),           # lanes sharing a column
+    ("u16", dict(bt=2, bw=64, bh=16, degrid=0.0), 330, 50),
+    ("u8", dict(bt=-1, sharpen=0.5, bw=8, bh=8), 120, 40),
+])
+def test_emu_forward_process_equals_forward_then_process(emu_lib, kind, kw, W, H):
+    """fft3d_plane_forward_process (new frame transformed inside the sweep)
+    gives the same output frames and ring spectra as forward + process, bit
+    for bit, including out-of-order requests."""
+    from fft3dcl_amd.filter import Instance
+    bits, st = kind_info(kind)
+    planes = make_clip(14, W, H, 5, kind)
+    order = [0, 1, 2, 4, 3] if kw["bt"] != 0 else list(range(5))
+    res = []
+    for fuse in (True, False):
+        inst = Instance(1, W, H, bits, st, 5, lib=emu_lib, fuse=fuse, **kw)
+        out = np.zeros_like(planes[0])
+        specs = {}
+        for n in order:
+            inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
+            specs.update({fn: inst.objs[0].read_spectrum(fn) for fn in range(5)
+                          if inst.objs[0].has(fn) and fn not in specs})
+        inst.close()
+        res.append((out, specs))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1].keys() == res[1][1].keys()
+    for fn in res[0][1]:
+        assert np.array_equal(res[0][1][fn], res[1][1][fn])
+
+
+def test_emu_forward_process_errors(emu_lib):
+    planes = make_clip(15, 96, 64, 4, "u8")
+    pl = capi.Plane(capi.make_params(bt=3), 96, 64, 8, lib=emu_lib)
+    src = np.ascontiguousarray(planes[0][1])
+    out = np.zeros((64, 96), np.uint8)
+    with pytest.raises(capi.FFT3DError, match="missing spectra"):
+        pl.forward_process(1, src, [0, 1, 2], out)
+    assert not pl.has(1)  # nothing was left half-built in the ring
+    with pytest.raises(capi.FFT3DError, match="window length"):
+        pl.forward_process(1, src, [0, 1], out)
+    pl.forward(0, np.ascontiguousarray(planes[0][0]))
+    pl.forward(2, np.ascontiguousarray(planes[0][2]))
+    pl.forward_process(1, src, [0, 1, 2], out)
+    assert pl.has(1)
+    # fn outside the window: plain forward + process
+    pl.forward_process(3, np.ascontiguousarray(planes[0][3]), [0, 1, 2], out)
+    assert pl.has(3)
+    pl.close()

@@ -149,22 +149,80 @@ def test_spectrum_vs_oracle(gpu, oracle):
 
 def test_profiling_counts_launches(gpu):
     frames = make_plane_sequence(12, 640, 360, 5, "u8")
+    src = [lambda fn: np.ascontiguousarray(frames[fn])]
+    out = np.zeros((360, 640), np.uint8)
+    # default: the last missing frame of each window is analysed inside the sweep
     inst = Instance(1, 640, 360, 8, 0, 5, bt=3, lib=gpu)
     inst.objs[0].set_profiling(True)
-    out = np.zeros((360, 640), np.uint8)
     for k in range(5):
-        inst.get_frame(k, [lambda fn: np.ascontiguousarray(frames[fn])], [out])
+        inst.get_frame(k, src, [out])
     prof = inst.objs[0].profile()
-    # the fused sweep (+ seam combine) is the default path at half overlap
-    assert prof["filter_synth_sweep"][0] == 5 and prof["analysis"][0] == 5
+    fused = prof.get("forward_filter_synth_sweep", (0, 0))[0]
+    assert fused == 4 and prof["analysis"][0] == 1  # frame 0 alone, then one new frame a step
+    assert fused + prof.get("filter_synth_sweep", (0, 0))[0] == 5
     assert prof["combine_seams"][0] == 5
     assert all(ms > 0 for _, ms in prof.values())
+    inst.close()
+    inst = Instance(1, 640, 360, 8, 0, 5, bt=3, lib=gpu, fuse=False)
+    inst.objs[0].set_profiling(True)
+    for k in range(5):
+        inst.get_frame(k, src, [out])
+    prof = inst.objs[0].profile()
+    assert prof["filter_synth_sweep"][0] == 5 and prof["analysis"][0] == 5
     inst.objs[0].set_option(capi.Plane.OPT_SWEEP, 0)
     for k in range(5):
-        inst.get_frame(k, [lambda fn: np.ascontiguousarray(frames[fn])], [out])
+        inst.get_frame(k, src, [out])
     prof = inst.objs[0].profile()
     assert prof["filter_inverse"][0] == 5 and prof["synthesize"][0] == 5
     inst.close()
+
+
+@pytest.mark.parametrize("kind,kw,W,H", [
+    ("u10", dict(bt=4, sharpen=0.4, sigma=3.0), 1920, 1080),
+    ("u8", dict(bt=3), 1000, 700),
+    ("f32", dict(bt=0, kratio=1.5, bw=64, bh=64), 1300, 700),
+    ("u8", dict(bt=1, bw=16, bh=64), 900, 500),
+    ("u16", dict(bt=2, bw=64, bh=16, degrid=0.0), 700, 300),
+])
+def test_forward_process_equals_forward_then_process(gpu, kind, kw, W, H):
+    """The new frame transformed inside the sweep: output frames and ring
+    spectra bit-identical to forward + process, out-of-order requests too."""
+    bits, st = kind_info(kind)
+    planes = make_clip(16, W, H, 5, kind)
+    order = [0, 1, 2, 4, 3] if kw["bt"] != 0 else list(range(5))
+    res = []
+    for fuse in (True, False):
+        inst = Instance(1, W, H, bits, st, 5, lib=gpu, fuse=fuse, **kw)
+        out = np.zeros_like(planes[0])
+        specs = {}
+        for n in order:
+            inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
+            specs.update({fn: inst.objs[0].read_spectrum(fn) for fn in range(5)
+                          if inst.objs[0].has(fn) and fn not in specs})
+        inst.close()
+        res.append((out, specs))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1].keys() == res[1][1].keys()
+    for fn in res[0][1]:
+        assert np.array_equal(res[0][1][fn], res[1][1][fn])
+
+
+def test_forward_process_device_buffers(gpu):
+    """Fused path reading the new plane in place from device memory and
+    writing the output to device memory, on a torch stream."""
+    W, H, n = 1280, 720, 6
+    frames = make_plane_sequence(17, W, H, n, "u10")
+    host = FFT3DCL([frames], bits=10, bt=4, lib=gpu, fuse=False)[0]
+    dev_in = torch.from_numpy(frames).cuda()
+    dev_out = torch.zeros((n, H, W), dtype=torch.int16, device="cuda")
+    stream = torch.cuda.Stream()
+    inst = Instance(1, W, H, 10, 0, n, bt=4, lib=gpu)
+    for k in range(n):
+        inst.get_frame(k, [lambda fn: dev_in[fn]], [dev_out[k]], on_device=True,
+                       stream=stream.cuda_stream)
+    stream.synchronize()
+    inst.close()
+    assert np.array_equal(dev_out.cpu().numpy().view(np.uint16), host)
 
 
 @pytest.mark.parametrize("kind,kw,W,H,R", [
