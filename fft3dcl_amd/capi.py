@@ -203,8 +203,8 @@ class Plane:
 
     def forward_process(self, fn: int, src, window, dst, src_on_device: bool = False,
                         dst_on_device: bool = False, stream: int | None = None):
-        """forward(fn, src) then process(window, dst) in one call; frame fn is
-        transformed inside the fused sweep when the geometry runs it."""
+        """forward(fn, src) then process(window, dst) in one call; with
+        OPT_FUSED_ANALYSIS set, frame fn is transformed inside the sweep."""
         w = (C.c_int * len(window))(*window)
         self._check(self.lib.fft3d_plane_forward_process(
             self.h, fn, _ptr(src), self.nbytes, int(src_on_device), w, len(window), _ptr(dst),
@@ -224,7 +224,7 @@ class Plane:
         self.lib.fft3d_plane_get_table(self.h, which, a.ctypes.data)
         return a
 
-    OPT_SWEEP, OPT_BAND_ROWS = 1, 2
+    OPT_SWEEP, OPT_BAND_ROWS, OPT_FUSED_ANALYSIS = 1, 2, 3
 
     def set_option(self, option: int, value: int):
         self._check(self.lib.fft3d_plane_set_option(self.h, option, int(value)))

@@ -33,6 +33,7 @@ struct KernelSet {
   AnalysisFn analysis = nullptr;
   FilterFn filter_inverse = nullptr;
   SweepFn sweep = nullptr;
+  SweepFn sweepFwd = nullptr;  // with the new frame's analysis inside
   int G = 0;
   int sweepThreads = 0;  // 64 * waves per sweep workgroup
   int nbw = 0;           // block columns per sweep strip
@@ -46,7 +47,8 @@ KernelSet make_set() {
   KernelSet k;
   k.analysis = &fk::k_analysis<BW, BH>;
   k.filter_inverse = &fk::k_filter_inverse<BW, BH>;
-  k.sweep = &fk::k_filter_synth<BW, BH, WPW>;
+  k.sweep = &fk::k_filter_synth<BW, BH, WPW, false>;
+  k.sweepFwd = &fk::k_filter_synth<BW, BH, WPW, true>;
   k.G = fk::Shape<BW, BH>::G;
   k.sweepThreads = 64 * WPW;
   k.nbw = fk::SweepShape<BW, BH, WPW>::NBW;
@@ -134,6 +136,7 @@ struct fft3d_plane {
   bool sweepOk = false;      // geometry allows it
   bool sweepOn = true;       // option
   int bandRows = 0;          // option, 0 = automatic
+  bool fuseOn = false;       // option: forward_process analyses inside the sweep
   float* dPart = nullptr;    // 4 partial planes
   int2* dSeams = nullptr;    // border tiles of the current band layout
   int nseams = 0, seamR = 0;
@@ -341,7 +344,7 @@ int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipS
     a.fwdPlane = fwdPlane;
     a.fwdSpec = fwdSpec;
     a.fwdMask = fwdMask;
-    SweepFn ksw = ks.sweep;
+    SweepFn ksw = fwdMask ? ks.sweepFwd : ks.sweep;
     hipEvent_t t0 = prof_begin(s);
     FK_LAUNCH(ksw, dim3(a.nstrips, a.nbands), dim3(ks.sweepThreads), s, args, w, dKalman, kfirst,
               a);
@@ -632,7 +635,7 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
   int mask = 0;
   for (int i = 0; i < nwin && i < 4; i++)
     if (window[i] == fn) mask |= 1 << i;
-  if (!mask || !(p->sweepOk && p->sweepOn)) {
+  if (!mask || !(p->sweepOk && p->sweepOn && p->fuseOn)) {
     const int rc = fft3d_plane_forward(p, fn, src, src_bytes, src_on_device, stream);
     if (rc != FFT3D_OK) return rc;
     return fft3d_plane_process(p, window, nwin, dst, dst_bytes, dst_on_device, stream);
@@ -698,6 +701,11 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
   case FFT3D_OPT_BAND_ROWS:
     if (value < 0) return p->fail(FFT3D_EINVAL, "band rows must be >= 0");
     p->bandRows = value;
+    return FFT3D_OK;
+  case FFT3D_OPT_FUSED_ANALYSIS:
+    if (value != 0 && value != 1)
+      return p->fail(FFT3D_EINVAL, "fused analysis option must be 0 or 1");
+    p->fuseOn = value == 1;
     return FFT3D_OK;
   }
   return p->fail(FFT3D_EINVAL, "unknown option");

@@ -36,8 +36,9 @@ struct SweepArgs {
   void* out;         // packed output plane (u8/u16/f32)
   const int2* seams; // border tiles (tile column, tile row)
   int nseams;
-  // Fused analysis of one new frame (fwdMask != 0): its plane, the ring
-  // spectrum to write, and the window slots that hold it (bit t = slot t).
+  // Fused analysis of one new frame (the FWD instance of the sweep): its
+  // plane, the ring spectrum to write, and the window slots that hold it
+  // (bit t = slot t).
   const void* fwdPlane;
   c32* fwdSpec;
   int fwdMask;
@@ -62,7 +63,7 @@ struct SweepShape {
   static constexpr int LDS_FLOATS = AREA + TAB;
 };
 
-template <int BW, int BH, int WPW>
+template <int BW, int BH, int WPW, bool FWD>
 __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
                                                            float4* kstate, int kfirst,
                                                            SweepArgs A) {
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   for (int m = m0; m < mEnd; m++) {
     const int b = m * P.ncols + n;
     __syncthreads();  // the previous row's strip reads are done: LDS is the block areas again
-    if (A.fwdMask) {
+    if (FWD) {
       // analysis of the new frame into the ring; the block DC also goes to
       // the last slot of Yb, which the forward transpose never touches
       c32* d = A.fwdSpec + (size_t)b * S::BSTR + j;
@@ -113,7 +114,8 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
                             });
       wave_sync_global();  // the ring stores are read back by the filter
     }
-    filter_block<BW, BH, true>(P, Wn, b, colLive, j, Yb, tab, kstate, kfirst != 0, A.fwdMask);
+    filter_block<BW, BH, true>(P, Wn, b, colLive, j, Yb, tab, kstate, kfirst != 0,
+                               FWD ? A.fwdMask : 0);
     float xr[BW], xi[BW];
     inverse_block<BW, BH>(colLive, j, Yb, xr, xi);
     const bool rowLane = colLive && j < S::NROW;

@@ -48,11 +48,10 @@ class Instance:
 
     def __init__(self, color_family: int, width: int, height: int, bits: int,
                  sample_type: int, num_frames: int, ssw: int = 1, ssh: int = 1,
-                 lib=None, fuse: bool = True, **kw):
-        """fuse: transform the last missing window frame inside the filter pass
-        (fft3d_plane_forward_process) instead of with its own forward call."""
+                 lib=None, fuse: bool = False, **kw):
+        """fuse: let the filter pass transform the last missing window frame
+        itself (FFT3D_OPT_FUSED_ANALYSIS) instead of a separate analysis."""
         device = int(kw.pop("device", 2))
-        self.fuse = fuse
         if device == 0:
             raise ValueError("FFT3DCL: device=0 selects the CPU backend, which this HIP "
                              "build does not ship (use device=1 or 2)")
@@ -71,6 +70,8 @@ class Instance:
             ph = height >> (ssh if q else 0)
             self.objs.append(capi.Plane(self.params, pw, ph, bits, sample_type,
                                         chroma=color_family == YUV and q > 0, lib=lib))
+        for o in self.objs:
+            o.set_option(capi.Plane.OPT_FUSED_ANALYSIS, int(fuse))
         info = self.objs[0].info
         self.bm, self.bt, self.current = info.bm, info.bt, info.current
 
@@ -90,16 +91,13 @@ class Instance:
             obj = self.objs[k]
             win = self.window(n)
             missing = [fn for fn in dict.fromkeys(win) if not obj.has(fn)]
-            if self.fuse and missing:
-                # the last missing frame is transformed inside the filter pass
-                for fn in missing[:-1]:
-                    obj.forward(fn, src_planes[q](fn), on_device, stream)
+            for fn in missing[:-1]:
+                obj.forward(fn, src_planes[q](fn), on_device, stream)
+            if missing:
                 fn = missing[-1]
                 obj.forward_process(fn, src_planes[q](fn), win, dst_planes[q], on_device,
                                     on_device, stream)
             else:
-                for fn in missing:
-                    obj.forward(fn, src_planes[q](fn), on_device, stream)
                 obj.process(win, dst_planes[q], on_device, stream)
             if self.bm >= 2:
                 obj.prune(n - self.current - 1, n - self.current + self.bt)

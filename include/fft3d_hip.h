@@ -115,10 +115,10 @@ int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst,
  * as one call: the per-frame sequence of the reference's scheduler, which
  * forwards each window frame missing from the cache and then processes the
  * window (src/fft3dcl.cpp:396-442). Same arguments, errors, ring contents and
- * output as the two calls. When fn is missing, belongs to the window and the
- * geometry runs the fused sweep, frame fn is transformed inside the sweep
- * (its spectrum still lands in the ring) instead of by its own launch, which
- * saves one spectrum write-then-read per frame. Otherwise it runs the two
+ * output as the two calls. With FFT3D_OPT_FUSED_ANALYSIS set, fn in the
+ * window and the geometry on the fused sweep, frame fn is transformed inside
+ * the sweep (its spectrum still lands in the ring) instead of by its own
+ * launch, which saves one spectrum read per frame. Otherwise it runs the two
  * calls. */
 int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t src_bytes,
                                 int src_on_device, const int* window, int nwin, void* dst,
@@ -158,8 +158,13 @@ int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max);
  *                        oh = bh/2); 0: separate filter/inverse and
  *                        synthesis kernels through a blocks buffer.
  *   FFT3D_OPT_BAND_ROWS  block rows per sweep band (0 = automatic).
+ *   FFT3D_OPT_FUSED_ANALYSIS  0 (default): fft3d_plane_forward_process runs
+ *                        the analysis launch then the sweep; 1: the sweep
+ *                        transforms the new frame itself (bit-identical,
+ *                        one launch and one spectrum read fewer, but slower
+ *                        on gfx950 so far: the sweep is latency-bound).
  * Returns FFT3D_EINVAL for an unknown option or value. */
-enum { FFT3D_OPT_SWEEP = 1, FFT3D_OPT_BAND_ROWS = 2 };
+enum { FFT3D_OPT_SWEEP = 1, FFT3D_OPT_BAND_ROWS = 2, FFT3D_OPT_FUSED_ANALYSIS = 3 };
 int fft3d_plane_set_option(fft3d_plane* p, int option, int value);
 
 /* Introspection for tests and tools. */

@@ -270,6 +270,9 @@ def test_emu_forward_process_equals_forward_then_process(emu_lib, kind, kw, W, H
 def test_emu_forward_process_errors(emu_lib):
     planes = make_clip(15, 96, 64, 4, "u8")
     pl = capi.Plane(capi.make_params(bt=3), 96, 64, 8, lib=emu_lib)
+    pl.set_option(capi.Plane.OPT_FUSED_ANALYSIS, 1)
+    with pytest.raises(capi.FFT3DError, match="must be 0 or 1"):
+        pl.set_option(capi.Plane.OPT_FUSED_ANALYSIS, 2)
     src = np.ascontiguousarray(planes[0][1])
     out = np.zeros((64, 96), np.uint8)
     with pytest.raises(capi.FFT3DError, match="missing spectra"):

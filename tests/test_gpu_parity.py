@@ -151,8 +151,8 @@ def test_profiling_counts_launches(gpu):
     frames = make_plane_sequence(12, 640, 360, 5, "u8")
     src = [lambda fn: np.ascontiguousarray(frames[fn])]
     out = np.zeros((360, 640), np.uint8)
-    # default: the last missing frame of each window is analysed inside the sweep
-    inst = Instance(1, 640, 360, 8, 0, 5, bt=3, lib=gpu)
+    # fused: the last missing frame of each window is analysed inside the sweep
+    inst = Instance(1, 640, 360, 8, 0, 5, bt=3, lib=gpu, fuse=True)
     inst.objs[0].set_profiling(True)
     for k in range(5):
         inst.get_frame(k, src, [out])
@@ -216,7 +216,7 @@ def test_forward_process_device_buffers(gpu):
     dev_in = torch.from_numpy(frames).cuda()
     dev_out = torch.zeros((n, H, W), dtype=torch.int16, device="cuda")
     stream = torch.cuda.Stream()
-    inst = Instance(1, W, H, 10, 0, n, bt=4, lib=gpu)
+    inst = Instance(1, W, H, 10, 0, n, bt=4, lib=gpu, fuse=True)
     for k in range(n):
         inst.get_frame(k, [lambda fn: dev_in[fn]], [dev_out[k]], on_device=True,
                        stream=stream.cuda_stream)
