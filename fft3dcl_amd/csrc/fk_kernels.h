@@ -244,12 +244,21 @@ __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* pl
   }
 }
 
+// Workgroup i of a launch runs on XCD i % 8, and each XCD has its own L2.
+// Mapping XCD x to a contiguous run of the n work items (a bijection on
+// [0, n)) keeps neighbouring blocks, which read overlapping pixel rows, under
+// one L2.
+__device__ __forceinline__ int xcd_contiguous(int i, int n) {
+  const int q = n / 8, r = n % 8, x = i % 8;
+  return x * q + min(x, r) + i / 8;
+}
+
 template <int BW, int BH>
 __global__ __launch_bounds__(64) void k_analysis(PlaneArgs P, const void* plane, c32* spec) {
   using S = Shape<BW, BH>;
   __shared__ c32 lds[S::LDS_C];
   const int lane = threadIdx.x, g = lane / S::LPB, j = lane - g * S::LPB;
-  const int b = blockIdx.x * S::G + g;
+  const int b = xcd_contiguous(blockIdx.x, gridDim.x) * S::G + g;
   const bool live = b < P.nb;
   c32* d = spec + (size_t)b * S::BSTR + j;
   forward_block<BW, BH>(P, plane, b, live, g, j, lds,
