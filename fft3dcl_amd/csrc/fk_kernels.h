@@ -287,12 +287,13 @@ __device__ __forceinline__ c32 wiener(c32 v, float thr, float sY, float betaX) {
 // Temporal filter + sharpen of one bin (src/cpu/reference.cpp:246-409,
 // src/cl/kernels.h:206-342). v[t] are the bin's values in temporal order,
 // d the degrid correction, ks the Kalman state of the bin (bm == 0).
+template <int BM>
 __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4], c32 d,
                                           float sX, float sY, float lut, float kcn, float4* ks,
                                           bool kfirst) {
   c32 f;
   sX = wiener_threshold(sX);
-  if (P.bm == 2) {
+  if constexpr (BM == 2) {
     const c32 pv = v[0], cu = v[1];
     c32 a = csub(cu, pv), bb = cadd(cu, pv);
     if (P.dg) bb = csub(bb, cscale(d, 2.0f));
@@ -300,7 +301,7 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
     bb = wiener(bb, sX, sY, P.betaX);
     f = cscale(cadd(a, bb), 0.5f);
     if (P.dg) f = cadd(f, d);
-  } else if (P.bm == 3) {
+  } else if constexpr (BM == 3) {
     const c32 pv = v[0], cu = v[1], nx = v[2];
     const float K = 0.8660254f;
     const c32 t1 = cadd(pv, nx);
@@ -314,7 +315,7 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
     d2 = wiener(d2, sX, sY, P.betaX);
     f = cscale(cadd(cadd(d0, d1), d2), 1.0f / 3.0f);
     if (P.dg) f = cadd(f, d);
-  } else if (P.bm == 4) {
+  } else if constexpr (BM == 4) {
     const c32 p2 = v[0], p1 = v[1], cu = v[2], nx = v[3];
     // 4-point temporal DFT as two radix-2 stages
     const c32 s02 = cadd(p2, cu), d02 = csub(cu, p2);
@@ -329,12 +330,12 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
     fn = wiener(fn, sX, sY, P.betaX);
     f = cscale(cadd(cadd(cadd(fp2, fp1), fc), fn), 0.25f);
     if (P.dg) f = cadd(f, d);
-  } else if (P.bm == 1) {
+  } else if constexpr (BM == 1) {
     c32 x = v[0];
     if (P.dg) x = csub(x, d);
     x = wiener(x, sX, sY, P.betaX);
     f = P.dg ? cadd(x, d) : x;
-  } else if (P.bm == 0) {
+  } else if constexpr (BM == 0) {
     // Kalman (src/cpu/reference.cpp:334-383); first frame initialises state.
     const c32 cu = v[0];
     float4 st = kfirst ? make_float4(cu.x, cu.y, kcn, kcn) : *ks;
@@ -430,7 +431,7 @@ __device__ __forceinline__ float4 col_factor(const float4* tab, int bh, int col,
 // repacked W'[r] = H0[r] + i Hh[r], H = (F[r] + conj F[-r]) / 2 the Hermitian
 // part that the real inverse transform keeps.
 // Both passes are called by every lane of the block (lb = lane in block).
-template <int BW, int BH, bool SEP>
+template <int BW, int BH, bool SEP, int BM>
 __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
@@ -438,59 +439,56 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   constexpr int FBW = S::FBW, NCOL = S::NCOL;
   constexpr int RS = S::LPB / NCOL;  // lanes sharing a column split its rows
   constexpr int NIT = BH / RS;       // rows per lane
-  constexpr int U = 4;
+  constexpr int U = NIT < 4 ? NIT : 4;  // rows per batch; divides NIT (powers of two)
+  constexpr int NB = NIT / U;
   const size_t base = (size_t)b * S::BSTR;
-  const int nwin = Wn.nwin;
   const int c = lb % NCOL, r0 = lb / NCOL;
   float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
   float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if constexpr (SEP) cf = col_factor(tab, BH, c, dc);
-  // loads of batch k+1 are in flight while batch k is filtered
+  // All four slots are loaded and staged: slots past nwin repeat a window
+  // spectrum (the host fills them), which keeps every load unconditional, so
+  // the next batch stays in flight while this one is filtered.
   auto load = [&](int i0, c32(&va)[U][4]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int r = r0 + (i0 + u) * RS;
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        va[u][t] = cmake(0.0f, 0.0f);
-        if (t < nwin && i0 + u < NIT)
-          va[u][t] = Wn.s[t][base + r * NCOL + c];
-      }
+      for (int t = 0; t < 4; t++) va[u][t] = Wn.s[t][base + r * NCOL + c];
     }
   };
   auto work = [&](int i0, const c32(&va)[U][4]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      if (i0 + u >= NIT) continue;
       const int r = r0 + (i0 + u) * RS;
       if (c == 0) {
 #pragma unroll
-        for (int t = 0; t < 4; t++)
-          if (t < nwin) Yb[S::st(t, r)] = va[u][t];
+        for (int t = 0; t < 4; t++) Yb[S::st(t, r)] = va[u][t];
       } else {
         BinConst kc;
         if constexpr (SEP) kc = bin_const_sep<BW>(P, r, c, tab[r], cf);
         else kc = bin_const<BW>(P, r, c, dc);
-        Yb[S::yidx(c, r)] = filter_bin(P, va[u], kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
+        Yb[S::yidx(c, r)] = filter_bin<BM>(P, va[u], kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                        kb ? kb + r * FBW + c : nullptr, kfirst);
       }
     }
   };
-  constexpr int NB = (NIT + U - 1) / U;
   c32 va[U][4], vb[U][4];
   load(0, va);
+  if constexpr (NB == 1) {
+    work(0, va);
+  } else {
 #pragma unroll 1
-  for (int k = 0; k < NB; k += 2) {
-    if (k + 1 < NB) load((k + 1) * U, vb);
-    work(k * U, va);
-    if (k + 1 < NB) {
+    for (int k = 0; k < NB; k += 2) {  // NB is even
+      load((k + 1) * U, vb);
+      work(k * U, va);
       if (k + 2 < NB) load((k + 2) * U, va);
       work((k + 1) * U, vb);
     }
   }
 }
 
-template <int BW, int BH, bool SEP>
+template <int BW, int BH, bool SEP, int BM>
 __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
@@ -524,7 +522,7 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
       BinConst kc;
       if constexpr (SEP) kc = bin_const_sep<BW>(P, r, col, tab[r], col == 0 ? cf0 : cfh);
       else kc = bin_const<BW>(P, r, col, dc);
-      const c32 y = filter_bin(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
+      const c32 y = filter_bin<BM>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                kb ? kb + r * FBW + col : nullptr, kfirst);
       // f[k] with a runtime k would live in scratch: select instead
       f[0] = k == 0 ? y : f[0];
@@ -545,6 +543,15 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
   }
 }
 
+template <int BW, int BH, bool SEP, int BM>
+__device__ __forceinline__ void filter_passes(const PlaneArgs& P, const WindowArgs& Wn, int b,
+                                              bool live, int lb, c32* Yb, const float4* tab,
+                                              float4* kstate, bool kfirst, float dc) {
+  if (live) filter_pass1<BW, BH, SEP, BM>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+  wave_sync();
+  if (live) filter_pass2<BW, BH, SEP, BM>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+}
+
 // Filter of block b into Yb; contains the wave barrier between its two passes,
 // so every lane of the wave calls it (live = this lane's block exists). Yb must
 // belong to the calling wave. With SEP the sharpen and degrid constants come
@@ -562,9 +569,15 @@ __device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArg
   if (live && P.dg)
     dc = ((fwdMask >> Wn.cur) & 1) ? Yb[S::YS - 1].x : Wn.s[Wn.cur][(size_t)b * S::BSTR].x;
   if (fwdMask) wave_sync();  // pass 1 stages over that slot
-  if (live) filter_pass1<BW, BH, SEP>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
-  wave_sync();
-  if (live) filter_pass2<BW, BH, SEP>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+  // one instance per temporal mode: only the plane's mode is in the hot loop
+  switch (P.bm) {
+  case 4: filter_passes<BW, BH, SEP, 4>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 3: filter_passes<BW, BH, SEP, 3>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 2: filter_passes<BW, BH, SEP, 2>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 1: filter_passes<BW, BH, SEP, 1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 0: filter_passes<BW, BH, SEP, 0>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  default: filter_passes<BW, BH, SEP, -1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc);
+  }
   wave_sync();
 }
 
