@@ -355,7 +355,7 @@ int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipS
     if (a.nseams) {
       CombineFn kc = &fk::k_combine_seams;
       t0 = prof_begin(s);
-      FK_LAUNCH(kc, dim3(a.nseams), dim3(256), s, args, a, ks.nbw);
+      FK_LAUNCH(kc, dim3((a.nseams + 3) / 4), dim3(256), s, args, a, ks.nbw);
       prof_end(t0, K_COMBINE, s);
       if (!launched("seam combine")) return FFT3D_ELAUNCH;
     }

@@ -204,10 +204,44 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   }
 }
 
-// Adds the partial sums of one border tile from every workgroup that touched
-// it, in (band, strip) order, adds chroma centring and packs.
+// Packs 4 consecutive samples (idx a multiple of 4) with store_sample's
+// arithmetic in one store.
+__device__ __forceinline__ void store_sample4(const PlaneArgs& P, void* out, size_t idx,
+                                              float4 v) {
+#pragma clang fp contract(off)
+  const float s[4] = {v.x, v.y, v.z, v.w};
+  if (P.pix == 8) {
+    unsigned w = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      float e = s[k] < 0.0f ? 0.0f : (s[k] > 1.0f ? 1.0f : s[k]);
+      w |= (unsigned)(uint8_t)floorf(e * 255.0f + 0.5f) << (8 * k);
+    }
+    *reinterpret_cast<unsigned*>(static_cast<uint8_t*>(out) + idx) = w;
+  } else if (P.pix == 16) {
+    unsigned w[2] = {0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      float e = s[k] * P.invNorm;
+      e = e < P.maxPix ? e : P.maxPix;
+      e = e < 0.0f ? 0.0f : (e > 1.0f ? 1.0f : e);
+      w[k >> 1] |= (unsigned)(uint16_t)floorf(e * 65535.0f + 0.5f) << (16 * (k & 1));
+    }
+    *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + idx) = make_uint2(w[0], w[1]);
+  } else {
+    *reinterpret_cast<float4*>(static_cast<float*>(out) + idx) = v;
+  }
+}
+
+// Adds the partial sums of border tiles from every workgroup that touched
+// them, in (band, strip) order, adds chroma centring and packs. One wave per
+// tile (so the contributor search is uniform), four samples per lane when the
+// tile width and the plane width allow 16-byte loads.
 __global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A, int nbw) {
-  const int2 tile = A.seams[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int si = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (si >= A.nseams) return;  // no barriers below
+  const int2 tile = A.seams[si];
   const int sx = P.bw / 2, sy = P.bh / 2;
   // bands touching tile row u: band i covers tile rows [i*R - 1, i*R + R - 1]
   int bands[2], nb = 0;
@@ -221,15 +255,41 @@ __global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A,
     if (s * nbw - 1 <= tile.x && tile.x <= s * nbw + nlive - 1)
       if (ns == 0 || strips[ns - 1] != s) strips[ns++] = s;
   }
-  for (int q = threadIdx.x; q < sx * sy; q += blockDim.x) {
-    const int x = tile.x * sx + q % sx, y = tile.y * sy + q / sx;
-    if (x >= P.W || y >= P.H) continue;
-    const size_t idx = (size_t)y * P.W + x;
-    float v = 0.0f;
-    for (int a = 0; a < nb; a++)
-      for (int c = 0; c < ns; c++)
-        v += A.part[(size_t)((bands[a] & 1) * 2 + (strips[c] & 1)) * P.W * P.H + idx];
-    store_sample(P, A.out, idx, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
+  const size_t plane = (size_t)P.W * P.H;
+  int planes[4], np = 0;
+  for (int a = 0; a < nb; a++)
+    for (int c = 0; c < ns; c++) planes[np++] = (bands[a] & 1) * 2 + (strips[c] & 1);
+  if (sx % 4 == 0 && P.W % 4 == 0) {
+    const int qpr = sx / 4;  // 4-sample groups per tile row
+    for (int q = lane; q < qpr * sy; q += 64) {
+      const int x = tile.x * sx + (q % qpr) * 4, y = tile.y * sy + q / qpr;
+      if (x >= P.W || y >= P.H) continue;
+      const size_t idx = (size_t)y * P.W + x;
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int k = 0; k < np; k++) {
+        const float4 u = *reinterpret_cast<const float4*>(A.part + planes[k] * plane + idx);
+        v.x += u.x;
+        v.y += u.y;
+        v.z += u.z;
+        v.w += u.w;
+      }
+      if (P.chromaNorm != 0.0f) {
+        v.x += P.chromaNorm;
+        v.y += P.chromaNorm;
+        v.z += P.chromaNorm;
+        v.w += P.chromaNorm;
+      }
+      store_sample4(P, A.out, idx, v);
+    }
+  } else {
+    for (int q = lane; q < sx * sy; q += 64) {
+      const int x = tile.x * sx + q % sx, y = tile.y * sy + q / sx;
+      if (x >= P.W || y >= P.H) continue;
+      const size_t idx = (size_t)y * P.W + x;
+      float v = 0.0f;
+      for (int k = 0; k < np; k++) v += A.part[planes[k] * plane + idx];
+      store_sample(P, A.out, idx, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
+    }
   }
 }
 

@@ -37,6 +37,10 @@ inline float4 make_float4(float x, float y, float z, float w) { return float4{x,
 struct uint4 {
   unsigned x, y, z, w;
 };
+struct uint2 {
+  unsigned x, y;
+};
+inline uint2 make_uint2(unsigned x, unsigned y) { return uint2{x, y}; }
 struct int2 {
   int x, y;
 };
