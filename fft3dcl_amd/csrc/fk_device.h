@@ -43,7 +43,7 @@ __device__ __forceinline__ float fsqrt(float x) {
 #if defined(FFT3D_CPU_EMU)
   return emu_sqrtf(x);
 #else
-  return __builtin_sqrtf(x);
+  return __builtin_amdgcn_sqrtf(x);
 #endif
 }
 

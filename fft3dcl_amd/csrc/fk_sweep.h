@@ -11,11 +11,11 @@
 // band one block row at a time:
 //   1. each wave filters + inverse-transforms its G blocks (fk_kernels.h),
 //      after transforming them for a frame being analysed in the same pass;
-//   2. the windowed blocks are overlap-added along the row in an LDS strip,
-//      even block columns storing and odd ones adding (fixed order: the
-//      result is bit-identical run to run, no atomics);
-//   3. the strip's upper half plus the lower half carried in registers from
-//      the previous block row completes one tile row: interior tiles are
+//   2. each block's windowed output goes to its own LDS area;
+//   3. one pass over the strip forms each pixel of a tile row: the two
+//      horizontally overlapping block halves of this row plus the lower half
+//      carried in registers from the previous block row (fixed order: the
+//      result is bit-identical run to run, no atomics); interior tiles are
 //      packed and stored to the output plane; tiles on the band's or strip's
 //      border are stored as float partial sums into one of four partial
 //      planes chosen by the (band, strip) parity, so neighbours never write
@@ -50,11 +50,12 @@ struct SweepShape {
   static constexpr int SX = BW / 2, SY = BH / 2;  // tile size
   static constexpr int NBW = S::G * WPW;          // block columns per strip
   static constexpr int CW = (NBW + 1) * SX;       // strip width in pixels
-  static constexpr int SP = CW + 1;               // strip pitch in floats
   static constexpr int NT = 64 * WPW;             // threads per workgroup
   static constexpr int NCARRY = (SY * CW + NT - 1) / NT;
   static constexpr int YFLOATS = 2 * S::G * S::YS * WPW;
-  static constexpr int AREA = ((YFLOATS > BH * SP ? YFLOATS : BH * SP) + 3) & ~3;
+  static constexpr int OP = BW + 1;  // row pitch of a windowed block in its LDS area
+  static_assert(BH * OP <= 2 * S::YS, "windowed block must fit its block area");
+  static constexpr int AREA = (YFLOATS + 3) & ~3;
   // constant tables kept in LDS after the block/strip area: the separable
   // sharpen/degrid factors (bh rows, then bw/2+1 columns, float4 each), then
   // the synthesis window factors synX (bw) and synY (bh)
@@ -69,7 +70,7 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
                                                            SweepArgs A) {
   using S = Shape<BW, BH>;
   using T = SweepShape<BW, BH, WPW>;
-  constexpr int SX = T::SX, SY = T::SY, NBW = T::NBW, CW = T::CW, SP = T::SP;
+  constexpr int SX = T::SX, SY = T::SY, NBW = T::NBW, CW = T::CW;
   __shared__ __attribute__((aligned(16))) float lds[T::LDS_FLOATS];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int g = lane / S::LPB, j = lane - g * S::LPB;
@@ -80,7 +81,6 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   const int n = n0 + w * S::G + g;  // this lane's block column
   const bool colLive = n < P.ncols;
   c32* Yb = reinterpret_cast<c32*>(lds) + (w * S::G + g) * S::YS;
-  float* strp = lds;
   float* part = A.part + (size_t)((band & 1) * 2 + (strip & 1)) * P.W * P.H;
   const int validCols = (nlive + 1) * SX;
   const int x0 = (n0 - 1) * SX;  // image column of strip column 0
@@ -129,35 +129,22 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
         xi[k] *= fx * fy1;
       }
     }
-    __syncthreads();  // every wave is done with its Y buffer: LDS becomes the strip
-    const int sc = (n - n0) * SX;
-    if (rowLane && (n & 1) == 0) {
-      float* r0 = strp + (2 * j) * SP + sc;
+    // windowed block -> its own LDS area as rows of OP floats (the area
+    // belongs to this wave, whose inverse reads of it are done)
+    wave_sync();
+    float* const blk = reinterpret_cast<float*>(Yb);
+    if (rowLane) {
+      float* o = blk + (2 * j) * T::OP;
 #pragma unroll
       for (int k = 0; k < BW; k++) {
-        r0[k] = xr[k];
-        r0[SP + k] = xi[k];
+        o[k] = xr[k];
+        o[T::OP + k] = xi[k];
       }
     }
-    __syncthreads();
-    if (rowLane && (n & 1) == 1) {
-      // left half always overlaps the even neighbour n-1 (same strip); the
-      // right half overlaps n+1 only when that block exists in the strip
-      const bool rightAdd = (n + 1 < n0 + nlive);
-      float* r0 = strp + (2 * j) * SP + sc;
-#pragma unroll
-      for (int k = 0; k < BW; k++) {
-        if (k < SX || rightAdd) {
-          r0[k] += xr[k];
-          r0[SP + k] += xi[k];
-        } else {
-          r0[k] = xr[k];
-          r0[SP + k] = xi[k];
-        }
-      }
-    }
-    __syncthreads();
-    // tile row m-1: carried lower half of row m-1 + upper half of row m
+    __syncthreads();  // every block of the strip is in LDS
+    // tile row m-1: carried lower half of row m-1 + upper half of row m.
+    // Strip column c lies in tile column tc = c / SX, covered by the right
+    // half of local block tc-1 and the left half of local block tc.
     {
       const int y0 = (m - 1) * SY;
       const bool firstRow = m == m0;
@@ -166,17 +153,33 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
       for (int i = 0; i < T::NCARRY; i++) {
         if (r < SY) {
           const bool ok = c < validCols;
-          const float top = ok ? strp[r * SP + c] : 0.0f;
+          const int tc = c / SX, x = c - tc * SX;
+          const int tl = tc > 0 ? tc - 1 : 0, tr = tc < nlive ? tc : 0;  // used when in range
+          const float* L = lds + (size_t)tl * 2 * S::YS + SX + x;  // block tc-1, right half
+          const float* Rb = lds + (size_t)tr * 2 * S::YS + x;      // block tc, left half
+          float top = 0.0f, bot = 0.0f;
+          if (ok) {
+            if (tc == 0) {
+              top = Rb[r * T::OP];
+              bot = Rb[(r + SY) * T::OP];
+            } else if (tc == nlive) {
+              top = L[r * T::OP];
+              bot = L[(r + SY) * T::OP];
+            } else {
+              top = L[r * T::OP] + Rb[r * T::OP];
+              bot = L[(r + SY) * T::OP] + Rb[(r + SY) * T::OP];
+            }
+          }
           const float v = firstRow ? top : carry[i] + top;
-          const int x = x0 + c, y = y0 + r;
-          if (ok && x >= 0 && x < P.W && y >= 0 && y < P.H) {
-            const size_t idx = (size_t)y * P.W + x;
+          const int xx = x0 + c, y = y0 + r;
+          if (ok && xx >= 0 && xx < P.W && y >= 0 && y < P.H) {
+            const size_t idx = (size_t)y * P.W + xx;
             if (firstRow || c < SX || c >= nlive * SX)
               part[idx] = v;  // border tile: partial sum
             else
               store_sample(P, A.out, idx, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
           }
-          carry[i] = ok ? strp[(r + SY) * SP + c] : 0.0f;
+          carry[i] = bot;
         }
         c += T::NT;
         while (c >= CW) {
