@@ -289,6 +289,7 @@ struct fft3d_plane {
     sw.ntx = ntx;
     sw.nty = nty;
     sw.part = dPart;
+    sw.ppitch = (d.width + 3) & ~3;
     sw.seams = dSeams;
     sw.nseams = nseams;
     return true;
@@ -510,7 +511,8 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
     ok = pl->alloc(reinterpret_cast<void**>(&pl->dKalman), sizeof(float4) * (size_t)pl->nb * nbin);
   pl->sweepOk = p.mode == 1 && p.ow * 2 == bw && p.oh * 2 == bh && sepMatches(pl->t, p);
   if (ok && pl->sweepOk) {
-    ok = pl->alloc(reinterpret_cast<void**>(&pl->dPart), sizeof(float) * 4 * (size_t)W * H);
+    ok = pl->alloc(reinterpret_cast<void**>(&pl->dPart),
+                   sizeof(float) * 4 * (size_t)((W + 3) & ~3) * H);
     ok = ok && pl->layoutSweep();
   }
   if (!ok) {

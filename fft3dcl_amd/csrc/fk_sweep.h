@@ -32,7 +32,8 @@ struct SweepArgs {
   int R;             // block rows per band
   int nstrips, nbands;
   int ntx, nty;      // output tiles covering the plane
-  float* part;       // 4 partial planes of W*H floats
+  float* part;       // 4 partial planes of H rows of ppitch floats
+  int ppitch;        // row pitch of the partial planes: W rounded up to 4
   void* out;         // packed output plane (u8/u16/f32)
   const int2* seams; // border tiles (tile column, tile row)
   int nseams;
@@ -53,7 +54,13 @@ struct SweepShape {
   static constexpr int NT = 64 * WPW;             // threads per workgroup
   static constexpr int NCARRY = (SY * CW + NT - 1) / NT;
   static constexpr int YFLOATS = 2 * S::G * S::YS * WPW;
-  static constexpr int OP = BW + 1;  // row pitch of a windowed block in its LDS area
+  // row pitch of a windowed block in its LDS area (16-byte aligned rows)
+  static constexpr int OP = BW + 4;
+  // tile-row pass: 4-sample quads when the tile width allows, else samples
+  static constexpr bool QUADS = SX % 4 == 0;
+  static constexpr int QW = CW / 4;                      // quads per strip row
+  static constexpr int NQ = (SY * QW + NT - 1) / NT;     // quads per thread
+  static constexpr int NCF = QUADS && 4 * NQ > NCARRY ? 4 * NQ : NCARRY;  // carry floats
   static_assert(BH * OP <= 2 * S::YS, "windowed block must fit its block area");
   static constexpr int AREA = (YFLOATS + 3) & ~3;
   // constant tables kept in LDS after the block/strip area: the separable
@@ -63,6 +70,42 @@ struct SweepShape {
   static constexpr int TAB = NFAC + BW + BH;
   static constexpr int LDS_FLOATS = AREA + TAB;
 };
+
+// Stores the first n <= 4 of 4 consecutive samples at idx with store_sample's
+// arithmetic: one packed store when all 4 are wanted and idx is a multiple of
+// 4, else sample by sample.
+__device__ __forceinline__ void store_sample4(const PlaneArgs& P, void* out, size_t idx,
+                                              float4 v, int n) {
+#pragma clang fp contract(off)
+  const float s[4] = {v.x, v.y, v.z, v.w};
+  if (n < 4 || (idx & 3) != 0) {
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (k < n) store_sample(P, out, idx + k, s[k]);
+    return;
+  }
+  if (P.pix == 8) {
+    unsigned w = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      float e = s[k] < 0.0f ? 0.0f : (s[k] > 1.0f ? 1.0f : s[k]);
+      w |= (unsigned)(uint8_t)floorf(e * 255.0f + 0.5f) << (8 * k);
+    }
+    *reinterpret_cast<unsigned*>(static_cast<uint8_t*>(out) + idx) = w;
+  } else if (P.pix == 16) {
+    unsigned w[2] = {0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      float e = s[k] * P.invNorm;
+      e = e < P.maxPix ? e : P.maxPix;
+      e = e < 0.0f ? 0.0f : (e > 1.0f ? 1.0f : e);
+      w[k >> 1] |= (unsigned)(uint16_t)floorf(e * 65535.0f + 0.5f) << (16 * (k & 1));
+    }
+    *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + idx) = make_uint2(w[0], w[1]);
+  } else {
+    *reinterpret_cast<float4*>(static_cast<float*>(out) + idx) = v;
+  }
+}
 
 template <int BW, int BH, int WPW, bool FWD>
 __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
@@ -81,10 +124,10 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   const int n = n0 + w * S::G + g;  // this lane's block column
   const bool colLive = n < P.ncols;
   c32* Yb = reinterpret_cast<c32*>(lds) + (w * S::G + g) * S::YS;
-  float* part = A.part + (size_t)((band & 1) * 2 + (strip & 1)) * P.W * P.H;
+  float* part = A.part + (size_t)((band & 1) * 2 + (strip & 1)) * A.ppitch * P.H;
   const int validCols = (nlive + 1) * SX;
   const int x0 = (n0 - 1) * SX;  // image column of strip column 0
-  float carry[T::NCARRY];
+  float carry[T::NCF];
   // constant tables (visible to every wave after the row loop's first barrier)
   float* tabf = lds + T::AREA;
   for (int i = tid; i < T::TAB; i += T::NT) {
@@ -145,7 +188,69 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
     // tile row m-1: carried lower half of row m-1 + upper half of row m.
     // Strip column c lies in tile column tc = c / SX, covered by the right
     // half of local block tc-1 and the left half of local block tc.
-    {
+    if constexpr (T::QUADS) {
+      // 4 samples per step: 16-byte LDS reads and stores (x0 is a multiple
+      // of 4; a quad crossing the plane's right edge stores its inside part)
+      const int y0 = (m - 1) * SY;
+      const bool firstRow = m == m0;
+      int r = tid / T::QW, q = tid - (tid / T::QW) * T::QW;
+#pragma unroll
+      for (int i = 0; i < T::NQ; i++) {
+        if (r < SY) {
+          const int c = 4 * q;
+          const bool ok = c < validCols;
+          const int tc = c / SX, x = c - tc * SX;
+          const int tl = tc > 0 ? tc - 1 : 0, tr = tc < nlive ? tc : 0;
+          const float* L = lds + (size_t)tl * 2 * S::YS + SX + x;
+          const float* Rb = lds + (size_t)tr * 2 * S::YS + x;
+          float4 top = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bot = top;
+          if (ok) {
+            if (tc == 0) {
+              top = *reinterpret_cast<const float4*>(Rb + r * T::OP);
+              bot = *reinterpret_cast<const float4*>(Rb + (r + SY) * T::OP);
+            } else if (tc == nlive) {
+              top = *reinterpret_cast<const float4*>(L + r * T::OP);
+              bot = *reinterpret_cast<const float4*>(L + (r + SY) * T::OP);
+            } else {
+              const float4 a0 = *reinterpret_cast<const float4*>(L + r * T::OP);
+              const float4 b0 = *reinterpret_cast<const float4*>(Rb + r * T::OP);
+              const float4 a1 = *reinterpret_cast<const float4*>(L + (r + SY) * T::OP);
+              const float4 b1 = *reinterpret_cast<const float4*>(Rb + (r + SY) * T::OP);
+              top = make_float4(a0.x + b0.x, a0.y + b0.y, a0.z + b0.z, a0.w + b0.w);
+              bot = make_float4(a1.x + b1.x, a1.y + b1.y, a1.z + b1.z, a1.w + b1.w);
+            }
+          }
+          float4 v = top;
+          if (!firstRow)
+            v = make_float4(carry[4 * i] + top.x, carry[4 * i + 1] + top.y,
+                            carry[4 * i + 2] + top.z, carry[4 * i + 3] + top.w);
+          const int xx = x0 + c, y = y0 + r;
+          if (ok && xx >= 0 && xx < P.W && y >= 0 && y < P.H) {
+            if (firstRow || c < SX || c >= nlive * SX) {
+              // border tile: partial sums (the pitch keeps the quad aligned)
+              *reinterpret_cast<float4*>(part + (size_t)y * A.ppitch + xx) = v;
+            } else {
+              if (P.chromaNorm != 0.0f) {
+                v.x += P.chromaNorm;
+                v.y += P.chromaNorm;
+                v.z += P.chromaNorm;
+                v.w += P.chromaNorm;
+              }
+              store_sample4(P, A.out, (size_t)y * P.W + xx, v, min(4, P.W - xx));
+            }
+          }
+          carry[4 * i] = bot.x;
+          carry[4 * i + 1] = bot.y;
+          carry[4 * i + 2] = bot.z;
+          carry[4 * i + 3] = bot.w;
+        }
+        q += T::NT;
+        while (q >= T::QW) {
+          q -= T::QW;
+          r++;
+        }
+      }
+    } else {
       const int y0 = (m - 1) * SY;
       const bool firstRow = m == m0;
       int r = tid / CW, c = tid - (tid / CW) * CW;  // element tid + NT*i, advanced below
@@ -173,11 +278,11 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
           const float v = firstRow ? top : carry[i] + top;
           const int xx = x0 + c, y = y0 + r;
           if (ok && xx >= 0 && xx < P.W && y >= 0 && y < P.H) {
-            const size_t idx = (size_t)y * P.W + xx;
             if (firstRow || c < SX || c >= nlive * SX)
-              part[idx] = v;  // border tile: partial sum
+              part[(size_t)y * A.ppitch + xx] = v;  // border tile: partial sum
             else
-              store_sample(P, A.out, idx, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
+              store_sample(P, A.out, (size_t)y * P.W + xx,
+                           P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
           }
           carry[i] = bot;
         }
@@ -190,14 +295,29 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
     }
   }
   // tile row mEnd-1: lower half of the band's last block row (a border row)
-  {
+  if constexpr (T::QUADS) {
+    const int y0 = (mEnd - 1) * SY;
+    int r = tid / T::QW, q = tid - (tid / T::QW) * T::QW;
+#pragma unroll
+    for (int i = 0; i < T::NQ; i++) {
+      const int c = 4 * q, x = x0 + c, y = y0 + r;
+      if (r < SY && c < validCols && x >= 0 && x < P.W && y >= 0 && y < P.H)
+        *reinterpret_cast<float4*>(part + (size_t)y * A.ppitch + x) =
+            make_float4(carry[4 * i], carry[4 * i + 1], carry[4 * i + 2], carry[4 * i + 3]);
+      q += T::NT;
+      while (q >= T::QW) {
+        q -= T::QW;
+        r++;
+      }
+    }
+  } else {
     const int y0 = (mEnd - 1) * SY;
     int r = tid / CW, c = tid - (tid / CW) * CW;
 #pragma unroll
     for (int i = 0; i < T::NCARRY; i++) {
       const int x = x0 + c, y = y0 + r;
       if (r < SY && c < validCols && x >= 0 && x < P.W && y >= 0 && y < P.H)
-        part[(size_t)y * P.W + x] = carry[i];
+        part[(size_t)y * A.ppitch + x] = carry[i];
       c += T::NT;
       while (c >= CW) {
         c -= CW;
@@ -207,39 +327,10 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   }
 }
 
-// Packs 4 consecutive samples (idx a multiple of 4) with store_sample's
-// arithmetic in one store.
-__device__ __forceinline__ void store_sample4(const PlaneArgs& P, void* out, size_t idx,
-                                              float4 v) {
-#pragma clang fp contract(off)
-  const float s[4] = {v.x, v.y, v.z, v.w};
-  if (P.pix == 8) {
-    unsigned w = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      float e = s[k] < 0.0f ? 0.0f : (s[k] > 1.0f ? 1.0f : s[k]);
-      w |= (unsigned)(uint8_t)floorf(e * 255.0f + 0.5f) << (8 * k);
-    }
-    *reinterpret_cast<unsigned*>(static_cast<uint8_t*>(out) + idx) = w;
-  } else if (P.pix == 16) {
-    unsigned w[2] = {0u, 0u};
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      float e = s[k] * P.invNorm;
-      e = e < P.maxPix ? e : P.maxPix;
-      e = e < 0.0f ? 0.0f : (e > 1.0f ? 1.0f : e);
-      w[k >> 1] |= (unsigned)(uint16_t)floorf(e * 65535.0f + 0.5f) << (16 * (k & 1));
-    }
-    *reinterpret_cast<uint2*>(static_cast<uint16_t*>(out) + idx) = make_uint2(w[0], w[1]);
-  } else {
-    *reinterpret_cast<float4*>(static_cast<float*>(out) + idx) = v;
-  }
-}
-
 // Adds the partial sums of border tiles from every workgroup that touched
 // them, in (band, strip) order, adds chroma centring and packs. One wave per
 // tile (so the contributor search is uniform), four samples per lane when the
-// tile width and the plane width allow 16-byte loads.
+// tile width is a multiple of 4.
 __global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A, int nbw) {
   const int lane = threadIdx.x & 63;
   const int si = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -258,19 +349,19 @@ __global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A,
     if (s * nbw - 1 <= tile.x && tile.x <= s * nbw + nlive - 1)
       if (ns == 0 || strips[ns - 1] != s) strips[ns++] = s;
   }
-  const size_t plane = (size_t)P.W * P.H;
+  const size_t plane = (size_t)A.ppitch * P.H;
   int planes[4], np = 0;
   for (int a = 0; a < nb; a++)
     for (int c = 0; c < ns; c++) planes[np++] = (bands[a] & 1) * 2 + (strips[c] & 1);
-  if (sx % 4 == 0 && P.W % 4 == 0) {
+  if (sx % 4 == 0) {
     const int qpr = sx / 4;  // 4-sample groups per tile row
     for (int q = lane; q < qpr * sy; q += 64) {
       const int x = tile.x * sx + (q % qpr) * 4, y = tile.y * sy + q / qpr;
       if (x >= P.W || y >= P.H) continue;
-      const size_t idx = (size_t)y * P.W + x;
+      const size_t pidx = (size_t)y * A.ppitch + x;
       float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       for (int k = 0; k < np; k++) {
-        const float4 u = *reinterpret_cast<const float4*>(A.part + planes[k] * plane + idx);
+        const float4 u = *reinterpret_cast<const float4*>(A.part + planes[k] * plane + pidx);
         v.x += u.x;
         v.y += u.y;
         v.z += u.z;
@@ -282,16 +373,16 @@ __global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A,
         v.z += P.chromaNorm;
         v.w += P.chromaNorm;
       }
-      store_sample4(P, A.out, idx, v);
+      store_sample4(P, A.out, (size_t)y * P.W + x, v, min(4, P.W - x));
     }
   } else {
     for (int q = lane; q < sx * sy; q += 64) {
       const int x = tile.x * sx + q % sx, y = tile.y * sy + q / sx;
       if (x >= P.W || y >= P.H) continue;
-      const size_t idx = (size_t)y * P.W + x;
+      const size_t pidx = (size_t)y * A.ppitch + x;
       float v = 0.0f;
-      for (int k = 0; k < np; k++) v += A.part[planes[k] * plane + idx];
-      store_sample(P, A.out, idx, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
+      for (int k = 0; k < np; k++) v += A.part[planes[k] * plane + pidx];
+      store_sample(P, A.out, (size_t)y * P.W + x, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
     }
   }
 }
