@@ -86,6 +86,9 @@ struct Shape {
   static constexpr int PITCH = NCOL + 3;
   static constexpr int YS = PITCH * BH;
   static constexpr int LDS_C = (G * NROW * ZP > G * YS) ? G * NROW * ZP : G * YS;
+  // k_analysis: the transpose in two rounds
+  static constexpr int SPLIT_A = NROW >= 2 ? 2 : 1;
+  static constexpr int LDS_A = G * (NROW / SPLIT_A) * ZP;
   __device__ __forceinline__ static int yidx(int c, int r) { return r * PITCH + c; }
   __device__ __forceinline__ static int st(int t, int r) {
     return r * PITCH + (t == 0 ? 0 : NCOL + t - 1);
@@ -180,16 +183,18 @@ __device__ __forceinline__ void load_row(const PlaneArgs& P, const void* plane, 
 
 // Windowed 2-D R2C FFT of block b. Lanes j < NCOL end with packed column j
 // of the spectrum and call emit(yr, yi) with it (r < BH). All lanes of the wave
-// call forward_block; its LDS area belongs to the wave. emit may write the
-// LDS area: within a wave the transpose reads precede it in issue order.
-template <int BW, int BH, class Emit>
+// call forward_block; its LDS area (NROW/SPLIT rows of ZP complex per block)
+// belongs to the wave. With SPLIT > 1 the row-to-column transpose goes through
+// it in SPLIT rounds, which divides the LDS a workgroup needs.
+template <int BW, int BH, int SPLIT = 1, class Emit>
 __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* plane, int b,
                                               bool live, int g, int j, c32* lds, Emit&& emit) {
   using S = Shape<BW, BH>;
+  constexpr int HR = S::NROW / SPLIT;  // row pairs per round through LDS
+  float zr[BW], zi[BW];
   if (live && j < S::NROW) {
     int x0, y0, par;
     block_origin(P, b, x0, y0, par);
-    float zr[BW], zi[BW];
     const int r0 = 2 * j;
     load_row<BW>(P, plane, x0, mirror(y0 + r0, P.H), zr);
     load_row<BW>(P, plane, x0, mirror(y0 + r0 + 1, P.H), zi);
@@ -211,34 +216,44 @@ __device__ __forceinline__ void forward_block(const PlaneArgs& P, const void* pl
       }
     }
     fft_reg<BW, false>(zr, zi);
-    c32* Z = lds + (g * S::NROW + j) * S::ZP;
-#pragma unroll
-    for (int k = 0; k < BW; k++) Z[k] = cmake(zr[k], zi[k]);
   }
-  wave_sync();
-  if (live && j < S::NCOL) {
-    const int c = j;
-    float yr[BH], yi[BH];
-    const c32* Zg = lds + g * S::NROW * S::ZP;
+  // row pairs [h*HR, (h+1)*HR) go through the block's LDS rows in round h
+  float yr[BH], yi[BH];
+  const c32* Zg = lds + g * HR * S::ZP;
 #pragma unroll
-    for (int p = 0; p < S::NROW; p++) {
-      const c32* Z = Zg + p * S::ZP;
-      if (c == 0) {
-        // rows 2p, 2p+1 are real at DC and Nyquist: X = (Re Z, Im Z)
-        const c32 a = Z[0], h = Z[BW / 2];
-        yr[2 * p] = a.x;
-        yi[2 * p] = h.x;
-        yr[2 * p + 1] = a.y;
-        yi[2 * p + 1] = h.y;
-      } else {
-        // X_2p[c] = (Z[c] + conj Z[-c]) / 2, X_2p+1[c] = (Z[c] - conj Z[-c]) / 2i
-        const c32 zc = Z[c], zm = Z[BW - c];
-        yr[2 * p] = 0.5f * (zc.x + zm.x);
-        yi[2 * p] = 0.5f * (zc.y - zm.y);
-        yr[2 * p + 1] = 0.5f * (zc.y + zm.y);
-        yi[2 * p + 1] = -0.5f * (zc.x - zm.x);
+  for (int h = 0; h < SPLIT; h++) {
+    if (live && j < S::NROW && j / HR == h) {
+      c32* Z = lds + (g * HR + j - h * HR) * S::ZP;
+#pragma unroll
+      for (int k = 0; k < BW; k++) Z[k] = cmake(zr[k], zi[k]);
+    }
+    wave_sync();
+    if (live && j < S::NCOL) {
+      const int c = j;
+#pragma unroll
+      for (int pp = 0; pp < HR; pp++) {
+        const int p = h * HR + pp;
+        const c32* Z = Zg + pp * S::ZP;
+        if (c == 0) {
+          // rows 2p, 2p+1 are real at DC and Nyquist: X = (Re Z, Im Z)
+          const c32 a = Z[0], hh = Z[BW / 2];
+          yr[2 * p] = a.x;
+          yi[2 * p] = hh.x;
+          yr[2 * p + 1] = a.y;
+          yi[2 * p + 1] = hh.y;
+        } else {
+          // X_2p[c] = (Z[c] + conj Z[-c]) / 2, X_2p+1[c] = (Z[c] - conj Z[-c]) / 2i
+          const c32 zc = Z[c], zm = Z[BW - c];
+          yr[2 * p] = 0.5f * (zc.x + zm.x);
+          yi[2 * p] = 0.5f * (zc.y - zm.y);
+          yr[2 * p + 1] = 0.5f * (zc.y + zm.y);
+          yi[2 * p + 1] = -0.5f * (zc.x - zm.x);
+        }
       }
     }
+    if (h + 1 < SPLIT) wave_sync();  // the next round overwrites the rows
+  }
+  if (live && j < S::NCOL) {
     fft_reg<BH, false>(yr, yi);
     emit(yr, yi);
   }
@@ -256,12 +271,12 @@ __device__ __forceinline__ int xcd_contiguous(int i, int n) {
 template <int BW, int BH>
 __global__ __launch_bounds__(64) void k_analysis(PlaneArgs P, const void* plane, c32* spec) {
   using S = Shape<BW, BH>;
-  __shared__ c32 lds[S::LDS_C];
+  __shared__ c32 lds[S::LDS_A];
   const int lane = threadIdx.x, g = lane / S::LPB, j = lane - g * S::LPB;
   const int b = xcd_contiguous(blockIdx.x, gridDim.x) * S::G + g;
   const bool live = b < P.nb;
   c32* d = spec + (size_t)b * S::BSTR + j;
-  forward_block<BW, BH>(P, plane, b, live, g, j, lds,
+  forward_block<BW, BH, S::SPLIT_A>(P, plane, b, live, g, j, lds,
                         [&](const float(&yr)[BH], const float(&yi)[BH]) {
 #pragma unroll
                           for (int r = 0; r < BH; r++) d[r * S::NCOL] = cmake(yr[r], yi[r]);

@@ -149,7 +149,7 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
       // analysis of the new frame into the ring; the block DC also goes to
       // the last slot of Yb, which the forward transpose never touches
       c32* d = A.fwdSpec + (size_t)b * S::BSTR + j;
-      forward_block<BW, BH>(P, A.fwdPlane, b, colLive, 0, j, Yb,
+      forward_block<BW, BH, S::SPLIT_A>(P, A.fwdPlane, b, colLive, 0, j, Yb,
                             [&](const float(&yr)[BH], const float(&yi)[BH]) {
 #pragma unroll
                               for (int r = 0; r < BH; r++) d[r * S::NCOL] = cmake(yr[r], yi[r]);
