@@ -20,9 +20,11 @@ HIP_LIB = PKG / "libfft3d_hip.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for every HIP source of this package.
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops"]
+# translation units: the C ABI, and the kernels of each block width
+BLOCK_WIDTHS = (4, 8, 16, 32, 64)
 
 
 def _run(cmd: list[str]) -> None:
@@ -33,10 +35,29 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_hip(out: Path = HIP_LIB) -> Path:
-    src = CSRC / "fft3d_capi.hip"
-    tmp = out.with_suffix(".so.tmp")
-    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", str(src), "-o", str(tmp)])
-    os.replace(tmp, out)
+    """Compiles the C ABI unit and one kernel unit per block width in
+    parallel, then links them into libfft3d_hip.so."""
+    import tempfile
+    inc = [f"-I{INCLUDE}", f"-I{CSRC}"]
+    with tempfile.TemporaryDirectory(prefix="fft3d_build_") as tmpdir:
+        units = [(CSRC / "fft3d_capi.hip", [], Path(tmpdir) / "capi.o")]
+        units += [(CSRC / "fk_set_bw.hip", [f"-DFK_BW={bw}"], Path(tmpdir) / f"bw{bw}.o")
+                  for bw in BLOCK_WIDTHS]
+        procs = [(subprocess.Popen([HIPCC, *HIP_FLAGS, *inc, *d, "-c", str(src), "-o", str(o)],
+                                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True),
+                  src) for src, d, o in units]
+        failed = []
+        for p, src in procs:
+            log, _ = p.communicate()
+            if p.returncode != 0:
+                failed.append(f"{src.name}:\n{log}")
+        if failed:
+            sys.stderr.write("\n".join(failed))
+            raise RuntimeError("build failed: " + ", ".join(f.split(":")[0] for f in failed))
+        tmp = out.with_suffix(".so.tmp")
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC",
+              *[str(o) for _, _, o in units], "-o", str(tmp)])
+        os.replace(tmp, out)
     return out
 
 

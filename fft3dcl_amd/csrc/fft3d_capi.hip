@@ -5,7 +5,8 @@
 // by design: tables and grid are built here from the validated parameters
 // (no CPU backend object is needed); the Kalman first-frame init is folded
 // into the filter launch; spectra are stored packed (fk_kernels.h).
-#include "fk_sweep.h"
+#define FK_COMMON_KERNELS  // k_synthesize and k_combine_seams live in this unit
+#include "fk_set.h"
 #include "fft3d_params.h"
 #include "fft3d_hip.h"
 
@@ -23,57 +24,20 @@ namespace {
 std::atomic<int> g_live{0};
 std::atomic<size_t> g_bytes{0};
 
-using AnalysisFn = void (*)(fk::PlaneArgs, const void*, fk::c32*);
-using FilterFn = void (*)(fk::PlaneArgs, fk::WindowArgs, float4*, int, float*);
-using SynthFn = void (*)(fk::PlaneArgs, const float*, void*);
-using SweepFn = void (*)(fk::PlaneArgs, fk::WindowArgs, float4*, int, fk::SweepArgs);
-using CombineFn = void (*)(fk::PlaneArgs, fk::SweepArgs, int);
-
-struct KernelSet {
-  AnalysisFn analysis = nullptr;
-  FilterFn filter_inverse = nullptr;
-  SweepFn sweep = nullptr;
-  SweepFn sweepFwd = nullptr;  // with the new frame's analysis inside
-  int G = 0;
-  int sweepThreads = 0;  // 64 * waves per sweep workgroup
-  int nbw = 0;           // block columns per sweep strip
-};
-
-// Sweep workgroups hold WPW waves; 4 unless the per-wave LDS of large blocks
-// would leave room for only one workgroup per CU.
-template <int BW, int BH>
-KernelSet make_set() {
-  constexpr int WPW = (BW * BH >= 4096) ? 2 : 4;
-  KernelSet k;
-  k.analysis = &fk::k_analysis<BW, BH>;
-  k.filter_inverse = &fk::k_filter_inverse<BW, BH>;
-  k.sweep = &fk::k_filter_synth<BW, BH, WPW, false>;
-  k.sweepFwd = &fk::k_filter_synth<BW, BH, WPW, true>;
-  k.G = fk::Shape<BW, BH>::G;
-  k.sweepThreads = 64 * WPW;
-  k.nbw = fk::SweepShape<BW, BH, WPW>::NBW;
-  return k;
-}
-
-template <int BW>
-KernelSet set_for_bh(int bh) {
-  switch (bh) {
-  case 4: return make_set<BW, 4>();
-  case 8: return make_set<BW, 8>();
-  case 16: return make_set<BW, 16>();
-  case 32: return make_set<BW, 32>();
-  case 64: return make_set<BW, 64>();
-  }
-  return KernelSet{};
-}
+using fk::AnalysisFn;
+using fk::CombineFn;
+using fk::FilterFn;
+using fk::KernelSet;
+using fk::SweepFn;
+using fk::SynthFn;
 
 KernelSet kernels_for(int bw, int bh) {
   switch (bw) {
-  case 4: return set_for_bh<4>(bh);
-  case 8: return set_for_bh<8>(bh);
-  case 16: return set_for_bh<16>(bh);
-  case 32: return set_for_bh<32>(bh);
-  case 64: return set_for_bh<64>(bh);
+  case 4: return fk::kernel_set<4>(bh);
+  case 8: return fk::kernel_set<8>(bh);
+  case 16: return fk::kernel_set<16>(bh);
+  case 32: return fk::kernel_set<32>(bh);
+  case 64: return fk::kernel_set<64>(bh);
   }
   return KernelSet{};
 }

@@ -689,6 +689,7 @@ __device__ __forceinline__ void store_sample(const PlaneArgs& P, void* out, size
   }
 }
 
+#ifdef FK_COMMON_KERNELS
 // Overlap-add of the windowed blocks covering each pixel, in block-index
 // order (src/cpu/reference.cpp:415-436), plus chroma centring, then pack.
 __global__ __launch_bounds__(256) void k_synthesize(PlaneArgs P, const float* blocks,
@@ -727,5 +728,6 @@ __global__ __launch_bounds__(256) void k_synthesize(PlaneArgs P, const float* bl
   if (P.chromaNorm != 0.0f) sum += P.chromaNorm;
   store_sample(P, out, (size_t)y * P.W + x, sum);
 }
+#endif  // FK_COMMON_KERNELS
 
 } // namespace fk

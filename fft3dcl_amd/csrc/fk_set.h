@@ -1,0 +1,32 @@
+// Kernel entry points of one block shape, as launched by the C ABI
+// (fft3d_capi.hip). The templates are instantiated per block width in
+// fk_set_bw.hip, one translation unit per width, so that they compile in
+// parallel.
+#pragma once
+
+#include "fk_sweep.h"
+
+namespace fk {
+
+using AnalysisFn = void (*)(PlaneArgs, const void*, c32*);
+using FilterFn = void (*)(PlaneArgs, WindowArgs, float4*, int, float*);
+using SynthFn = void (*)(PlaneArgs, const float*, void*);
+using SweepFn = void (*)(PlaneArgs, WindowArgs, float4*, int, SweepArgs);
+using CombineFn = void (*)(PlaneArgs, SweepArgs, int);
+
+struct KernelSet {
+  AnalysisFn analysis = nullptr;
+  FilterFn filter_inverse = nullptr;
+  SweepFn sweep = nullptr;
+  SweepFn sweepFwd = nullptr;  // with the new frame's analysis inside
+  int G = 0;
+  int sweepThreads = 0;  // 64 * waves per sweep workgroup
+  int nbw = 0;           // block columns per sweep strip
+};
+
+// The kernels of block shape (BW, bh); an empty set for an unsupported bh.
+// Defined for BW in {4, 8, 16, 32, 64} by fk_set_bw.hip.
+template <int BW>
+KernelSet kernel_set(int bh);
+
+}  // namespace fk

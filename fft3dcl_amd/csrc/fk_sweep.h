@@ -327,6 +327,7 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   }
 }
 
+#ifdef FK_COMMON_KERNELS
 // Adds the partial sums of border tiles from every workgroup that touched
 // them, in (band, strip) order, adds chroma centring and packs. One wave per
 // tile (so the contributor search is uniform), four samples per lane when the
@@ -386,5 +387,6 @@ __global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A,
     }
   }
 }
+#endif  // FK_COMMON_KERNELS
 
 }  // namespace fk
