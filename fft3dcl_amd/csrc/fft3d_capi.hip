@@ -518,6 +518,7 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   a.pat = p.usePattern ? 1 : 0;
   a.shp = p.sharpenOn ? 1 : 0;
   a.sigmaX = p.beta * p.wienerS;
+  a.thrX = fk::wiener_threshold(a.sigmaX);
   a.sigmaY = p.wienerS;
   a.betaX = p.betaX;
   a.beta = p.beta;

@@ -42,6 +42,7 @@ struct PlaneArgs {
   int bm;              // temporal mode -1..4
   int dg, pat, shp;    // degrid, sigma pattern, sharpen on/off
   float sigmaX, sigmaY, betaX, beta;
+  float thrX;  // wiener_threshold(sigmaX): the Wiener test is psd > thrX
   float ssmin, ssmax, kalmanCN, kr2;
   const float* winAna;  // bw*bh analysis window
   const float* winSynA; // bw*bh synthesis window, even block columns (mode 0: winAna)
@@ -287,8 +288,9 @@ __global__ __launch_bounds__(64) void k_analysis(PlaneArgs P, const void* plane,
 
 // Wiener gain (src/cpu/reference.cpp:201-209). The reference's test
 // 1/sigmaX > 1/psd is evaluated as psd > thr with thr = wiener_threshold(sX)
+// (computed once on the host, PlaneArgs::thrX, unless a pattern varies sX)
 // (no reciprocal per call); the gain is continuous at the threshold.
-__device__ __forceinline__ float wiener_threshold(float sX) {
+__host__ __device__ inline float wiener_threshold(float sX) {
   if (sX > 0.0f) return sX;
   // 1/(+0) = +inf passes every psd; 1/(-0) and 1/negative never pass.
   return (sX == 0.0f && copysignf(1.0f, sX) > 0.0f) ? -1.0f : __builtin_inff();
@@ -299,23 +301,30 @@ __device__ __forceinline__ c32 wiener(c32 v, float thr, float sY, float betaX) {
   return cscale(v, mul);
 }
 
+// A feature flag: fixed at compile time (F = 0 or 1) or read from the plane
+// (F = 2).
+template <int F>
+__device__ __forceinline__ bool flag(int v) {
+  return F == 2 ? v != 0 : F != 0;
+}
+
 // Temporal filter + sharpen of one bin (src/cpu/reference.cpp:246-409,
 // src/cl/kernels.h:206-342). v[t] are the bin's values in temporal order,
 // d the degrid correction, ks the Kalman state of the bin (bm == 0).
-template <int BM>
+template <int BM, int DG, int SHP>
 __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4], c32 d,
                                           float sX, float sY, float lut, float kcn, float4* ks,
                                           bool kfirst) {
+  const bool dg = flag<DG>(P.dg), shp = flag<SHP>(P.shp);
   c32 f;
-  sX = wiener_threshold(sX);
   if constexpr (BM == 2) {
     const c32 pv = v[0], cu = v[1];
     c32 a = csub(cu, pv), bb = cadd(cu, pv);
-    if (P.dg) bb = csub(bb, cscale(d, 2.0f));
+    if (dg) bb = csub(bb, cscale(d, 2.0f));
     a = wiener(a, sX, sY, P.betaX);
     bb = wiener(bb, sX, sY, P.betaX);
     f = cscale(cadd(a, bb), 0.5f);
-    if (P.dg) f = cadd(f, d);
+    if (dg) f = cadd(f, d);
   } else if constexpr (BM == 3) {
     const c32 pv = v[0], cu = v[1], nx = v[2];
     const float K = 0.8660254f;
@@ -324,12 +333,12 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
     const c32 dn = csub(nx, pv);
     const c32 y2 = cmake(K * dn.y, -K * dn.x);
     c32 d0 = cadd(y1, y2), d1 = cadd(cu, t1), d2 = csub(y1, y2);
-    if (P.dg) d1 = csub(d1, cscale(d, 3.0f));
+    if (dg) d1 = csub(d1, cscale(d, 3.0f));
     d0 = wiener(d0, sX, sY, P.betaX);
     d1 = wiener(d1, sX, sY, P.betaX);
     d2 = wiener(d2, sX, sY, P.betaX);
     f = cscale(cadd(cadd(d0, d1), d2), 1.0f / 3.0f);
-    if (P.dg) f = cadd(f, d);
+    if (dg) f = cadd(f, d);
   } else if constexpr (BM == 4) {
     const c32 p2 = v[0], p1 = v[1], cu = v[2], nx = v[3];
     // 4-point temporal DFT as two radix-2 stages
@@ -338,18 +347,18 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
     c32 fc = cadd(s02, s13), fp2 = csub(s02, s13);
     c32 fp1 = cmake(d02.x - d13.y, d02.y + d13.x);  // (c - p2) + i (n - p1)
     c32 fn = cmake(d02.x + d13.y, d02.y - d13.x);   // (c - p2) - i (n - p1)
-    if (P.dg) fp1 = csub(fp1, cscale(d, 4.0f));
+    if (dg) fp1 = csub(fp1, cscale(d, 4.0f));
     fp2 = wiener(fp2, sX, sY, P.betaX);
     fp1 = wiener(fp1, sX, sY, P.betaX);
     fc = wiener(fc, sX, sY, P.betaX);
     fn = wiener(fn, sX, sY, P.betaX);
     f = cscale(cadd(cadd(cadd(fp2, fp1), fc), fn), 0.25f);
-    if (P.dg) f = cadd(f, d);
+    if (dg) f = cadd(f, d);
   } else if constexpr (BM == 1) {
     c32 x = v[0];
-    if (P.dg) x = csub(x, d);
+    if (dg) x = csub(x, d);
     x = wiener(x, sX, sY, P.betaX);
-    f = P.dg ? cadd(x, d) : x;
+    f = dg ? cadd(x, d) : x;
   } else if constexpr (BM == 0) {
     // Kalman (src/cpu/reference.cpp:334-383); first frame initialises state.
     const c32 cu = v[0];
@@ -372,13 +381,13 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
   } else {
     f = v[0];  // bm == -1: the current frame passes through
   }
-  if (P.shp) {
-    c32 x = P.dg ? csub(f, d) : f;
+  if (shp) {
+    c32 x = dg ? csub(f, d) : f;
     const float psd = x.x * x.x + x.y * x.y + 1e-37f;
     const float q = psd * P.ssmax * frcp((psd + P.ssmin) * (psd + P.ssmax));
     const float gsh = 1.0f + lut * fsqrt(q);
     x = cscale(x, gsh);
-    f = P.dg ? cadd(x, d) : x;
+    f = dg ? cadd(x, d) : x;
   }
   return f;
 }
@@ -392,36 +401,37 @@ struct BinConst {
 __device__ __forceinline__ void bin_sigma(const PlaneArgs& P, int i, BinConst& k) {
   if (P.pat) {
     const float pv = P.pattern[i];
-    k.sX = pv * P.beta;
+    k.sX = wiener_threshold(pv * P.beta);
     k.sY = pv;
     k.kcn = pv;
   } else {
-    k.sX = P.sigmaX;
+    k.sX = P.thrX;
     k.sY = P.sigmaY;
     k.kcn = P.kalmanCN;
   }
 }
 // From the full sharpen and degrid tables in device memory.
-template <int BW>
+template <int BW, int DG, int SHP>
 __device__ __forceinline__ BinConst bin_const(const PlaneArgs& P, int r, int col, float dc) {
   constexpr int FBW = BW / 2 + 1;
   BinConst k;
   const int i = r * FBW + col;
   bin_sigma(P, i, k);
-  k.lut = P.shp ? P.sharpen[i] : 0.0f;
-  k.d = P.dg ? cscale(P.gs[i], dc) : cmake(0.0f, 0.0f);
+  k.lut = flag<SHP>(P.shp) ? P.sharpen[i] : 0.0f;
+  k.d = flag<DG>(P.dg) ? cscale(P.gs[i], dc) : cmake(0.0f, 0.0f);
   return k;
 }
 // From the separable factors: rf = row factor {shY, gyRe, gyIm, strength},
 // cf = column factor with its degrid part already scaled by the block DC.
-template <int BW>
+template <int BW, int DG, int SHP>
 __device__ __forceinline__ BinConst bin_const_sep(const PlaneArgs& P, int r, int col, float4 rf,
                                                   float4 cf) {
   constexpr int FBW = BW / 2 + 1;
   BinConst k;
   bin_sigma(P, r * FBW + col, k);
-  k.lut = P.shp ? rf.w - rf.x * cf.x : 0.0f;
-  k.d = P.dg ? cmake(cf.y * rf.y - cf.z * rf.z, cf.y * rf.z + cf.z * rf.y) : cmake(0.0f, 0.0f);
+  k.lut = flag<SHP>(P.shp) ? rf.w - rf.x * cf.x : 0.0f;
+  k.d = flag<DG>(P.dg) ? cmake(cf.y * rf.y - cf.z * rf.z, cf.y * rf.z + cf.z * rf.y)
+                       : cmake(0.0f, 0.0f);
   return k;
 }
 // Column factor of column col scaled for a block of DC dc.
@@ -446,7 +456,7 @@ __device__ __forceinline__ float4 col_factor(const float4* tab, int bh, int col,
 // repacked W'[r] = H0[r] + i Hh[r], H = (F[r] + conj F[-r]) / 2 the Hermitian
 // part that the real inverse transform keeps.
 // Both passes are called by every lane of the block (lb = lane in block).
-template <int BW, int BH, bool SEP, int BM>
+template <int BW, int BH, bool SEP, int BM, int DG, int SHP>
 __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
@@ -481,9 +491,9 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
         for (int t = 0; t < 4; t++) Yb[S::st(t, r)] = va[u][t];
       } else {
         BinConst kc;
-        if constexpr (SEP) kc = bin_const_sep<BW>(P, r, c, tab[r], cf);
-        else kc = bin_const<BW>(P, r, c, dc);
-        Yb[S::yidx(c, r)] = filter_bin<BM>(P, va[u], kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
+        if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP>(P, r, c, tab[r], cf);
+        else kc = bin_const<BW, DG, SHP>(P, r, c, dc);
+        Yb[S::yidx(c, r)] = filter_bin<BM, DG, SHP>(P, va[u], kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                        kb ? kb + r * FBW + c : nullptr, kfirst);
       }
     }
@@ -503,7 +513,7 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   }
 }
 
-template <int BW, int BH, bool SEP, int BM>
+template <int BW, int BH, bool SEP, int BM, int DG, int SHP>
 __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
@@ -535,9 +545,9 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
                         : cmake(0.5f * (wa.y + wb.y), -0.5f * (wa.x - wb.x));
       }
       BinConst kc;
-      if constexpr (SEP) kc = bin_const_sep<BW>(P, r, col, tab[r], col == 0 ? cf0 : cfh);
-      else kc = bin_const<BW>(P, r, col, dc);
-      const c32 y = filter_bin<BM>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
+      if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP>(P, r, col, tab[r], col == 0 ? cf0 : cfh);
+      else kc = bin_const<BW, DG, SHP>(P, r, col, dc);
+      const c32 y = filter_bin<BM, DG, SHP>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                kb ? kb + r * FBW + col : nullptr, kfirst);
       // f[k] with a runtime k would live in scratch: select instead
       f[0] = k == 0 ? y : f[0];
@@ -558,13 +568,35 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
   }
 }
 
-template <int BW, int BH, bool SEP, int BM>
+template <int BW, int BH, bool SEP, int BM, int DG, int SHP>
 __device__ __forceinline__ void filter_passes(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                               bool live, int lb, c32* Yb, const float4* tab,
                                               float4* kstate, bool kfirst, float dc) {
-  if (live) filter_pass1<BW, BH, SEP, BM>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+  if (live) filter_pass1<BW, BH, SEP, BM, DG, SHP>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
   wave_sync();
-  if (live) filter_pass2<BW, BH, SEP, BM>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+  if (live) filter_pass2<BW, BH, SEP, BM, DG, SHP>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+}
+
+// The Wiener modes get one instance per (degrid, sharpen) pair; Kalman and
+// the one-frame modes read both flags at run time.
+template <int BW, int BH, bool SEP, int BM>
+__device__ __forceinline__ void filter_mode(const PlaneArgs& P, const WindowArgs& Wn, int b,
+                                            bool live, int lb, c32* Yb, const float4* tab,
+                                            float4* kstate, bool kfirst, float dc) {
+#define FK_PASSES(DG, SHP) \
+  filter_passes<BW, BH, SEP, BM, DG, SHP>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc)
+  if constexpr (BM >= 2) {
+    if (P.dg) {
+      if (P.shp) FK_PASSES(1, 1);
+      else FK_PASSES(1, 0);
+    } else {
+      if (P.shp) FK_PASSES(0, 1);
+      else FK_PASSES(0, 0);
+    }
+  } else {
+    FK_PASSES(2, 2);
+  }
+#undef FK_PASSES
 }
 
 // Filter of block b into Yb; contains the wave barrier between its two passes,
@@ -586,12 +618,12 @@ __device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArg
   if (fwdMask) wave_sync();  // pass 1 stages over that slot
   // one instance per temporal mode: only the plane's mode is in the hot loop
   switch (P.bm) {
-  case 4: filter_passes<BW, BH, SEP, 4>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  case 3: filter_passes<BW, BH, SEP, 3>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  case 2: filter_passes<BW, BH, SEP, 2>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  case 1: filter_passes<BW, BH, SEP, 1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  case 0: filter_passes<BW, BH, SEP, 0>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  default: filter_passes<BW, BH, SEP, -1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc);
+  case 4: filter_mode<BW, BH, SEP, 4>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 3: filter_mode<BW, BH, SEP, 3>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 2: filter_mode<BW, BH, SEP, 2>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 1: filter_mode<BW, BH, SEP, 1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 0: filter_mode<BW, BH, SEP, 0>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  default: filter_mode<BW, BH, SEP, -1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc);
   }
   wave_sync();
 }
