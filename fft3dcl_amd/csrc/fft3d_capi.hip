@@ -82,6 +82,7 @@ struct fft3d_plane {
   fft3d::Tables t;
   fft3d_plane_desc d{};
   int nb = 0, ncols = 0, nrows = 0, fbw = 0, bps = 4, pix = 0;
+  int ncu = 0;  // compute units of the plane's device
   size_t specBytes = 0;
   KernelSet ks;
   fk::PlaneArgs args{};
@@ -210,8 +211,10 @@ struct fft3d_plane {
     const int nstrips = (ncols + nbw - 1) / nbw;
     int R = bandRows;
     if (R <= 0) {
-      // about two workgroups per CU on a 256-CU part, at least 2 rows per band
-      int nbands = (512 + nstrips - 1) / nstrips;
+      // one round of workgroups: as many as the device's CUs hold at once,
+      // at least 2 rows per band
+      const int slots = (ncu > 0 ? ncu : 256) * std::max(1, ks.sweepPerCU);
+      int nbands = (slots + nstrips - 1) / nstrips;
       R = (nrows + nbands - 1) / nbands;
       if (R < 2) R = 2;
     }
@@ -422,6 +425,13 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   }
   const fft3d::Params& p = pl->p;
   pl->ks = kernels_for(p.bw, p.bh);
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      pl->ncu = prop.multiProcessorCount;
+    (void)hipGetLastError();
+  }
   if (!pl->ks.analysis) {
     delete pl;
     set_err(err, errlen, "FFT3DCL: GPU setup failed: block too large for the GPU path (max 64x64)");

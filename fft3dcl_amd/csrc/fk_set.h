@@ -22,6 +22,7 @@ struct KernelSet {
   int G = 0;
   int sweepThreads = 0;  // 64 * waves per sweep workgroup
   int nbw = 0;           // block columns per sweep strip
+  int sweepPerCU = 0;    // sweep workgroups one CU holds (LDS, 2 waves per SIMD)
 };
 
 // The kernels of block shape (BW, bh); an empty set for an unsupported bh.

@@ -153,7 +153,7 @@ inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
   std::strcpy(p->gcnArchName, "emu");
   p->sharedMemPerBlock = 160 * 1024;
   p->totalGlobalMem = (size_t)1 << 34;
-  p->multiProcessorCount = 1;
+  p->multiProcessorCount = 256;  // the emulated part: MI355X
   return hipSuccess;
 }
 inline hipError_t hipGetLastError() { return hipSuccess; }
