@@ -398,8 +398,9 @@ struct BinConst {
   float sX, sY, lut, kcn;
   c32 d;
 };
+template <int PAT>
 __device__ __forceinline__ void bin_sigma(const PlaneArgs& P, int i, BinConst& k) {
-  if (P.pat) {
+  if (flag<PAT>(P.pat)) {
     const float pv = P.pattern[i];
     k.sX = wiener_threshold(pv * P.beta);
     k.sY = pv;
@@ -411,24 +412,24 @@ __device__ __forceinline__ void bin_sigma(const PlaneArgs& P, int i, BinConst& k
   }
 }
 // From the full sharpen and degrid tables in device memory.
-template <int BW, int DG, int SHP>
+template <int BW, int DG, int SHP, int PAT>
 __device__ __forceinline__ BinConst bin_const(const PlaneArgs& P, int r, int col, float dc) {
   constexpr int FBW = BW / 2 + 1;
   BinConst k;
   const int i = r * FBW + col;
-  bin_sigma(P, i, k);
+  bin_sigma<PAT>(P, i, k);
   k.lut = flag<SHP>(P.shp) ? P.sharpen[i] : 0.0f;
   k.d = flag<DG>(P.dg) ? cscale(P.gs[i], dc) : cmake(0.0f, 0.0f);
   return k;
 }
 // From the separable factors: rf = row factor {shY, gyRe, gyIm, strength},
 // cf = column factor with its degrid part already scaled by the block DC.
-template <int BW, int DG, int SHP>
+template <int BW, int DG, int SHP, int PAT>
 __device__ __forceinline__ BinConst bin_const_sep(const PlaneArgs& P, int r, int col, float4 rf,
                                                   float4 cf) {
   constexpr int FBW = BW / 2 + 1;
   BinConst k;
-  bin_sigma(P, r * FBW + col, k);
+  bin_sigma<PAT>(P, r * FBW + col, k);
   k.lut = flag<SHP>(P.shp) ? rf.w - rf.x * cf.x : 0.0f;
   k.d = flag<DG>(P.dg) ? cmake(cf.y * rf.y - cf.z * rf.z, cf.y * rf.z + cf.z * rf.y)
                        : cmake(0.0f, 0.0f);
@@ -456,7 +457,7 @@ __device__ __forceinline__ float4 col_factor(const float4* tab, int bh, int col,
 // repacked W'[r] = H0[r] + i Hh[r], H = (F[r] + conj F[-r]) / 2 the Hermitian
 // part that the real inverse transform keeps.
 // Both passes are called by every lane of the block (lb = lane in block).
-template <int BW, int BH, bool SEP, int BM, int DG, int SHP>
+template <int BW, int BH, bool SEP, int BM, int DG, int SHP, int PAT>
 __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
@@ -491,8 +492,8 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
         for (int t = 0; t < 4; t++) Yb[S::st(t, r)] = va[u][t];
       } else {
         BinConst kc;
-        if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP>(P, r, c, tab[r], cf);
-        else kc = bin_const<BW, DG, SHP>(P, r, c, dc);
+        if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP, PAT>(P, r, c, tab[r], cf);
+        else kc = bin_const<BW, DG, SHP, PAT>(P, r, c, dc);
         Yb[S::yidx(c, r)] = filter_bin<BM, DG, SHP>(P, va[u], kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                        kb ? kb + r * FBW + c : nullptr, kfirst);
       }
@@ -513,7 +514,7 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   }
 }
 
-template <int BW, int BH, bool SEP, int BM, int DG, int SHP>
+template <int BW, int BH, bool SEP, int BM, int DG, int SHP, int PAT>
 __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
@@ -545,8 +546,8 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
                         : cmake(0.5f * (wa.y + wb.y), -0.5f * (wa.x - wb.x));
       }
       BinConst kc;
-      if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP>(P, r, col, tab[r], col == 0 ? cf0 : cfh);
-      else kc = bin_const<BW, DG, SHP>(P, r, col, dc);
+      if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP, PAT>(P, r, col, tab[r], col == 0 ? cf0 : cfh);
+      else kc = bin_const<BW, DG, SHP, PAT>(P, r, col, dc);
       const c32 y = filter_bin<BM, DG, SHP>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                kb ? kb + r * FBW + col : nullptr, kfirst);
       // f[k] with a runtime k would live in scratch: select instead
@@ -568,33 +569,39 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
   }
 }
 
-template <int BW, int BH, bool SEP, int BM, int DG, int SHP>
+template <int BW, int BH, bool SEP, int BM, int DG, int SHP, int PAT>
 __device__ __forceinline__ void filter_passes(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                               bool live, int lb, c32* Yb, const float4* tab,
                                               float4* kstate, bool kfirst, float dc) {
-  if (live) filter_pass1<BW, BH, SEP, BM, DG, SHP>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+  if (live)
+    filter_pass1<BW, BH, SEP, BM, DG, SHP, PAT>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
   wave_sync();
-  if (live) filter_pass2<BW, BH, SEP, BM, DG, SHP>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+  if (live)
+    filter_pass2<BW, BH, SEP, BM, DG, SHP, PAT>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
 }
 
-// The Wiener modes get one instance per (degrid, sharpen) pair; Kalman and
-// the one-frame modes read both flags at run time.
+// The Wiener modes get one instance per (degrid, sharpen, sigma pattern)
+// combination; Kalman and the one-frame modes read the flags at run time.
 template <int BW, int BH, bool SEP, int BM>
 __device__ __forceinline__ void filter_mode(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                             bool live, int lb, c32* Yb, const float4* tab,
                                             float4* kstate, bool kfirst, float dc) {
-#define FK_PASSES(DG, SHP) \
-  filter_passes<BW, BH, SEP, BM, DG, SHP>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc)
+#define FK_PASSES(DG, SHP, PAT) \
+  filter_passes<BW, BH, SEP, BM, DG, SHP, PAT>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc)
   if constexpr (BM >= 2) {
-    if (P.dg) {
-      if (P.shp) FK_PASSES(1, 1);
-      else FK_PASSES(1, 0);
-    } else {
-      if (P.shp) FK_PASSES(0, 1);
-      else FK_PASSES(0, 0);
+    const int sel = (P.dg ? 4 : 0) | (P.shp ? 2 : 0) | (P.pat ? 1 : 0);
+    switch (sel) {
+    case 7: FK_PASSES(1, 1, 1); break;
+    case 6: FK_PASSES(1, 1, 0); break;
+    case 5: FK_PASSES(1, 0, 1); break;
+    case 4: FK_PASSES(1, 0, 0); break;
+    case 3: FK_PASSES(0, 1, 1); break;
+    case 2: FK_PASSES(0, 1, 0); break;
+    case 1: FK_PASSES(0, 0, 1); break;
+    default: FK_PASSES(0, 0, 0);
     }
   } else {
-    FK_PASSES(2, 2);
+    FK_PASSES(2, 2, 2);
   }
 #undef FK_PASSES
 }
