@@ -20,7 +20,7 @@ reference's CPU path (its own sources built into oracle/_ref when present,
 else the oracle restatement) single-threaded on a bounded 3-frame sample of
 the same clip.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-host]
 """
 import argparse
 import json
@@ -78,12 +78,47 @@ def cpu_baseline(frames):
                       f"(4 analyses + 3 filter/synthesis), {dt:.1f} s single-threaded"}
 
 
+def host_buffer_rates(frames, steps=24):
+    """Frames/s of the same workload when the planes enter and leave through
+    host buffers (the VapourSynth plugin's boundary): PCIe included, both
+    directions, as the C ABI does it (host-to-device copy before the analysis,
+    device-to-host copy after the synthesis, the call returns when the host
+    side is done). Pageable numpy buffers, then pinned ones."""
+    import numpy as np
+    import torch
+    from fft3dcl_amd.filter import GRAY, Instance
+    res = {}
+    pinned = torch.from_numpy(frames).pin_memory()
+    out_pinned = torch.empty((H, W), dtype=torch.int16).pin_memory()
+    for kind in ("pageable", "pinned"):
+        inst = Instance(GRAY, W, H, BITS, 0, steps + 16, **PARAMS)
+        if kind == "pageable":
+            out = np.zeros((H, W), np.uint16)
+            srcs = [lambda fn: frames[fn % NUNIQ]]
+            dst = out
+        else:
+            srcs = [lambda fn: pinned[fn % NUNIQ]]
+            dst = out_pinned
+        for n in range(4):
+            inst.get_frame(n, srcs, [dst])
+        t0 = time.perf_counter()
+        for n in range(4, 4 + steps):
+            inst.get_frame(n, srcs, [dst])
+        dt = time.perf_counter() - t0
+        inst.close()
+        res[kind + "_fps"] = round(steps / dt, 1)
+    res["frames"] = steps
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) rate measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, --steps times (the "
                          "reference point for HBM byte counters)")
@@ -180,6 +215,8 @@ def main():
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
     }
+    if not args.no_host:
+        line["config"]["host_buffers_pcie_inclusive"] = host_buffer_rates(frames)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(frames)
     print(json.dumps(line), flush=True)
