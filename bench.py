@@ -20,7 +20,7 @@ reference's CPU path (its own sources built into oracle/_ref when present,
 else the oracle restatement) single-threaded on a bounded 3-frame sample of
 the same clip.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-host]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-host] [--no-served]
 """
 import argparse
 import json
@@ -111,6 +111,36 @@ def host_buffer_rates(frames, steps=24):
     return res
 
 
+def served_sample(seconds=0.5, nstreams=32, uniq=3):
+    """The served form (32 seeded streams of mixed resolution, bit depth, bt
+    and block size, Poisson arrivals at 24-60 frames/s each over `seconds`):
+    p50/p99 latency at that offered load, and the capacity of the same
+    requests issued all at once. Frames are seeded noise made on the GPU
+    (`uniq` per stream, cycled): the filter's cost does not depend on them."""
+    import torch
+    from fft3dcl_amd.serve import Server, make_specs, summarize
+    specs = make_specs(SEED, nstreams, duration=seconds)
+    g = torch.Generator(device="cuda").manual_seed(SEED)
+    src, out = {}, {}
+    for sp in specs:
+        dt = torch.uint8 if sp.bits == 8 else torch.int16
+        src[sp.sid] = torch.randint(0, 1 << sp.bits, (uniq, sp.height, sp.width), generator=g,
+                                    device="cuda", dtype=torch.int32).to(dt)
+        out[sp.sid] = torch.empty((sp.height, sp.width), dtype=dt, device="cuda")
+    sources = {sid: (lambda fn, t=t: t[fn % uniq]) for sid, t in src.items()}
+    budget = 64 << 30
+    Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=False)  # warm-up
+    cap = Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=False)
+    paced = Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=True)
+    offered = sum(sp.nframes for sp in specs) / seconds
+    res = {"offered_frames_per_s": round(offered, 1), "at_offered_load": summarize(specs, paced),
+           "capacity": summarize(specs, cap),
+           "mix": "seeded: resolutions 640x360..3840x2160, 8/10-bit, bt 0/2/3/4, bw 16/32/64"}
+    res["capacity"].pop("p50_ms")
+    res["capacity"].pop("p99_ms")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +149,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-buffer (PCIe-inclusive) rate measurement")
+    ap.add_argument("--no-served", action="store_true",
+                    help="skip the 32-stream served-form sample")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, --steps times (the "
                          "reference point for HBM byte counters)")
@@ -217,6 +249,8 @@ def main():
     }
     if not args.no_host:
         line["config"]["host_buffers_pcie_inclusive"] = host_buffer_rates(frames)
+    if not args.no_served:
+        line["config"]["served_32_streams"] = served_sample()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(frames)
     print(json.dumps(line), flush=True)

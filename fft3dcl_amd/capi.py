@@ -224,7 +224,7 @@ class Plane:
         self.lib.fft3d_plane_get_table(self.h, which, a.ctypes.data)
         return a
 
-    OPT_SWEEP, OPT_BAND_ROWS, OPT_FUSED_ANALYSIS = 1, 2, 3
+    OPT_SWEEP, OPT_BAND_ROWS, OPT_FUSED_ANALYSIS, OPT_RESERVE_SPECTRA = 1, 2, 3, 4
 
     def set_option(self, option: int, value: int):
         self._check(self.lib.fft3d_plane_set_option(self.h, option, int(value)))

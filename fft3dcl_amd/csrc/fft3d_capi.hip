@@ -679,6 +679,15 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
     if (value < 0) return p->fail(FFT3D_EINVAL, "band rows must be >= 0");
     p->bandRows = value;
     return FFT3D_OK;
+  case FFT3D_OPT_RESERVE_SPECTRA:
+    if (value < 0 || value > 64) return p->fail(FFT3D_EINVAL, "reserve count must be 0..64");
+    while ((int)(p->pool.size() + p->ring.size()) < value) {
+      fk::c32* m = nullptr;
+      if (!p->alloc(reinterpret_cast<void**>(&m), p->specBytes))
+        return p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed");
+      p->pool.push_back(m);
+    }
+    return FFT3D_OK;
   case FFT3D_OPT_FUSED_ANALYSIS:
     if (value != 0 && value != 1)
       return p->fail(FFT3D_EINVAL, "fused analysis option must be 0 or 1");

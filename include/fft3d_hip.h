@@ -163,8 +163,16 @@ int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max);
  *                        transforms the new frame itself (bit-identical,
  *                        one launch and one spectrum read fewer, but slower
  *                        on gfx950 so far: the sweep is latency-bound).
+ *   FFT3D_OPT_RESERVE_SPECTRA  allocates that many spectra into the reuse
+ *                        pool now (FFT3D_ENOMEM if they do not fit), so that
+ *                        a serving loop makes no device allocation later.
  * Returns FFT3D_EINVAL for an unknown option or value. */
-enum { FFT3D_OPT_SWEEP = 1, FFT3D_OPT_BAND_ROWS = 2, FFT3D_OPT_FUSED_ANALYSIS = 3 };
+enum {
+  FFT3D_OPT_SWEEP = 1,
+  FFT3D_OPT_BAND_ROWS = 2,
+  FFT3D_OPT_FUSED_ANALYSIS = 3,
+  FFT3D_OPT_RESERVE_SPECTRA = 4
+};
 int fft3d_plane_set_option(fft3d_plane* p, int option, int value);
 
 /* Introspection for tests and tools. */
