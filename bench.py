@@ -21,6 +21,7 @@ else the oracle restatement) single-threaded on a bounded 3-frame sample of
 the same clip.
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-host] [--no-served]
+                    [--no-variants]
 """
 import argparse
 import json
@@ -111,6 +112,51 @@ def host_buffer_rates(frames, steps=24):
     return res
 
 
+def fp16_ring_variant(dev, kb_fn, steps=100, warmup=10):
+    """The same workload with the spectra ring stored as FP16 (arithmetic
+    FP32): frames/s, per-kernel times (sampled as in the main run), and the
+    largest difference from the FP32 ring's output over 8 frames, in LSB."""
+    import numpy as np
+    import torch
+    from fft3dcl_amd.filter import GRAY, Instance
+    outs = {}
+    res = {}
+    for half in (False, True):
+        inst = Instance(GRAY, W, H, BITS, 0, steps + warmup + 16, half=half, **PARAMS)
+        plane = inst.objs[0]
+        stream = torch.cuda.Stream()
+        out = torch.empty((H, W), dtype=torch.int16, device="cuda")
+        srcs = [lambda fn: dev[fn % NUNIQ]]
+        frames = []
+        for n in range(8):
+            inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
+            with torch.cuda.stream(stream):  # ordered after the frame's kernels
+                frames.append(out.clone())
+        for n in range(8, 8 + warmup):
+            inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        plane.profile()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i, n in enumerate(range(8 + warmup, 8 + warmup + steps)):
+            plane.set_profiling(i % PROFILE_EVERY == 0)
+            inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
+        plane.set_profiling(False)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        prof = plane.profile()
+        info = plane.info
+        inst.close()
+        outs[half] = torch.stack(frames).cpu().numpy().view(np.uint16).astype(np.int32)
+        if half:
+            ms = e0.elapsed_time(e1) / steps
+            res = {"frames_per_s": round(1000.0 / ms, 1), "ms_per_step": round(ms, 4),
+                   "kernels_ms": {k: round(t / c, 4) for k, (c, t) in prof.items()},
+                   "ring_bytes_per_spectrum": info.spectrum_bytes}
+    res["max_lsb_vs_fp32_ring"] = int(np.abs(outs[True] - outs[False]).max())
+    return res
+
+
 def served_sample(seconds=0.5, nstreams=32, uniq=3):
     """The served form (32 seeded streams of mixed resolution, bit depth, bt
     and block size, Poisson arrivals at 24-60 frames/s each over `seconds`):
@@ -151,6 +197,8 @@ def main():
                     help="skip the host-buffer (PCIe-inclusive) rate measurement")
     ap.add_argument("--no-served", action="store_true",
                     help="skip the 32-stream served-form sample")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP16-spectra-ring variant of the workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, --steps times (the "
                          "reference point for HBM byte counters)")
@@ -247,6 +295,8 @@ def main():
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
     }
+    if not args.no_variants:
+        line["config"]["fp16_spectra_ring"] = fp16_ring_variant(dev, kernel_bytes)
     if not args.no_host:
         line["config"]["host_buffers_pcie_inclusive"] = host_buffer_rates(frames)
     if not args.no_served:

@@ -225,6 +225,7 @@ class Plane:
         return a
 
     OPT_SWEEP, OPT_BAND_ROWS, OPT_FUSED_ANALYSIS, OPT_RESERVE_SPECTRA = 1, 2, 3, 4
+    OPT_SPECTRA_FP16 = 5
 
     def set_option(self, option: int, value: int):
         self._check(self.lib.fft3d_plane_set_option(self.h, option, int(value)))

@@ -48,6 +48,24 @@ void set_err(char* err, size_t n, const std::string& s) {
   }
 }
 
+// IEEE binary16 -> float (host side of the FP16 spectra readback).
+float half_to_float(uint32_t h) {
+  const uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  uint32_t x;
+  if (e == 0) {
+    const float v = (float)m * 5.9604644775390625e-8f;  // m * 2^-24
+    std::memcpy(&x, &v, 4);
+    x |= sign;
+  } else if (e == 31) {
+    x = sign | 0x7f800000u | (m << 13);
+  } else {
+    x = sign | ((e - 15u + 127u) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
 // True when the separable factors reproduce the sharpen and degrid tables to
 // float rounding (the fused sweep builds the per-bin constants from them). A
 // zero scutoff gives NaN entries that no product of factors reproduces.
@@ -102,6 +120,7 @@ struct fft3d_plane {
   bool sweepOn = true;       // option
   int bandRows = 0;          // option, 0 = automatic
   bool fuseOn = false;       // option: forward_process analyses inside the sweep
+  bool half = false;         // option: spectra stored as FP16 pairs
   float* dPart = nullptr;    // 4 partial planes
   int2* dSeams = nullptr;    // border tiles of the current band layout
   int nseams = 0, seamR = 0;
@@ -192,6 +211,12 @@ struct fft3d_plane {
     }
     ring[fn] = m;
     return m;
+  }
+
+  void release(fk::c32* m) {
+    (void)hipFree(m);
+    bytes -= specBytes;
+    g_bytes -= specBytes;
   }
 
   void retire(fk::c32* m) {
@@ -312,7 +337,7 @@ int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipS
     a.fwdPlane = fwdPlane;
     a.fwdSpec = fwdSpec;
     a.fwdMask = fwdMask;
-    SweepFn ksw = fwdMask ? ks.sweepFwd : ks.sweep;
+    SweepFn ksw = fwdMask ? ks.sweepFwd : (half ? ks.sweepH : ks.sweep);
     hipEvent_t t0 = prof_begin(s);
     FK_LAUNCH(ksw, dim3(a.nstrips, a.nbands), dim3(ks.sweepThreads), s, args, w, dKalman, kfirst,
               a);
@@ -330,7 +355,7 @@ int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipS
   } else {
     if (fwdMask) return fail(FFT3D_EINVAL, "fused analysis needs the sweep path");
     const int grid = (nb + ks.G - 1) / ks.G;
-    FilterFn kf = ks.filter_inverse;
+    FilterFn kf = half ? ks.filterH : ks.filter_inverse;
     hipEvent_t t0 = prof_begin(s);
     FK_LAUNCH(kf, dim3(grid), dim3(64), s, args, w, dKalman, kfirst, dBlocks);
     prof_end(t0, K_FILTER, s);
@@ -584,7 +609,7 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
     plane = p->dStage;
   }
   const int grid = (p->nb + p->ks.G - 1) / p->ks.G;
-  AnalysisFn k = p->ks.analysis;
+  AnalysisFn k = p->half ? p->ks.analysisH : p->ks.analysis;
   hipEvent_t t0 = p->prof_begin(s);
   FK_LAUNCH(k, dim3(grid), dim3(64), s, p->args, plane, spec);
   p->prof_end(t0, fft3d_plane::K_ANALYSIS, s);
@@ -612,7 +637,7 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
   int mask = 0;
   for (int i = 0; i < nwin && i < 4; i++)
     if (window[i] == fn) mask |= 1 << i;
-  if (!mask || !(p->sweepOk && p->sweepOn && p->fuseOn)) {
+  if (!mask || !(p->sweepOk && p->sweepOn && p->fuseOn) || p->half) {
     const int rc = fft3d_plane_forward(p, fn, src, src_bytes, src_on_device, stream);
     if (rc != FFT3D_OK) return rc;
     return fft3d_plane_process(p, window, nwin, dst, dst_bytes, dst_on_device, stream);
@@ -679,6 +704,25 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
     if (value < 0) return p->fail(FFT3D_EINVAL, "band rows must be >= 0");
     p->bandRows = value;
     return FFT3D_OK;
+  case FFT3D_OPT_SPECTRA_FP16: {
+    if (value != 0 && value != 1)
+      return p->fail(FFT3D_EINVAL, "FP16 spectra option must be 0 or 1");
+    if (value == 1 && !p->ks.analysisH)
+      return p->fail(FFT3D_EUNSUPPORTED, "FP16 spectra need square blocks of 16, 32 or 64");
+    if ((value == 1) == p->half) return FFT3D_OK;
+    // the stored spectra change size: drop the ring and the pool
+    if (hipStreamSynchronize(p->own) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+      (void)hipGetLastError();
+      return p->fail(FFT3D_ELAUNCH, "synchronize failed");
+    }
+    for (auto& kv : p->ring) p->release(kv.second);
+    for (auto* m : p->pool) p->release(m);
+    p->ring.clear();
+    p->pool.clear();
+    p->half = value == 1;
+    p->specBytes = (p->half ? 4 : 8) * (size_t)p->nb * p->p.bh * (p->p.bw / 2);
+    return FFT3D_OK;
+  }
   case FFT3D_OPT_RESERVE_SPECTRA:
     if (value < 0 || value > 64) return p->fail(FFT3D_EINVAL, "reserve count must be 0..64");
     while ((int)(p->pool.size() + p->ring.size()) < value) {
@@ -784,11 +828,15 @@ int fft3d_plane_read_spectrum(fft3d_plane* p, int fn, float* dst) {
   if (it == p->ring.end()) return p->fail(FFT3D_EMISSING, "missing spectra");
   const int bw = p->p.bw, bh = p->p.bh, nc = bw / 2, fbw = p->fbw;
   std::vector<fk::c32> h((size_t)p->nb * bh * nc);
-  if (hipStreamSynchronize(p->own) != hipSuccess ||
-      hipMemcpy(h.data(), it->second, p->specBytes, hipMemcpyDeviceToHost) != hipSuccess) {
+  std::vector<uint32_t> hh(p->half ? h.size() : 0);
+  void* host = p->half ? static_cast<void*>(hh.data()) : static_cast<void*>(h.data());
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(host, it->second, p->specBytes, hipMemcpyDeviceToHost) != hipSuccess) {
     (void)hipGetLastError();
     return p->fail(FFT3D_ELAUNCH, "spectrum readback failed");
   }
+  for (size_t i = 0; i < hh.size(); i++)
+    h[i] = fk::c32{half_to_float(hh[i] & 0xffffu), half_to_float(hh[i] >> 16)};
   for (int b = 0; b < p->nb; b++) {
     const fk::c32* s = &h[(size_t)b * bh * nc];
     float* d = dst + (size_t)b * bh * fbw * 2;

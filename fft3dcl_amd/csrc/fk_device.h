@@ -30,6 +30,27 @@ __device__ __forceinline__ c32 csub(c32 a, c32 b) { return c32{a.x - b.x, a.y - 
 __device__ __forceinline__ c32 cconj(c32 a) { return c32{a.x, -a.y}; }
 __device__ __forceinline__ c32 cscale(c32 a, float s) { return c32{a.x * s, a.y * s}; }
 
+// Two floats as FP16 (round to nearest even) packed low-high in 32 bits, and
+// back: the storage of a half-precision spectra ring.
+__device__ __forceinline__ uint32_t pack_h2(float a, float b) {
+#if defined(FFT3D_CPU_EMU)
+  return (uint32_t)emu_f2h(a) | ((uint32_t)emu_f2h(b) << 16);
+#else
+  const _Float16 ha = (_Float16)a, hb = (_Float16)b;
+  return (uint32_t)__builtin_bit_cast(uint16_t, ha) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+#endif
+}
+__device__ __forceinline__ float2 unpack_h2(uint32_t v) {
+#if defined(FFT3D_CPU_EMU)
+  return float2{emu_h2f((uint16_t)(v & 0xffffu)), emu_h2f((uint16_t)(v >> 16))};
+#else
+  const _Float16 a = __builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+  const _Float16 b = __builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
+  return make_float2((float)a, (float)b);
+#endif
+}
+
 // Reciprocal and square root used by the filter. Tolerances of the path are
 // far above one ulp, so the hardware approximations (1 ulp) are used.
 __device__ __forceinline__ float frcp(float x) {

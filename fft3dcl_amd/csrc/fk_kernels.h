@@ -69,6 +69,32 @@ struct WindowArgs {
   int cur;   // index of the current frame in s[]
 };
 
+// Element i of a stored spectrum: complex FP32 (8 bytes), or with H the two
+// halves rounded to FP16 (4 bytes, round to nearest even) in the same layout.
+template <bool H>
+__device__ __forceinline__ c32 spec_load(const c32* s, size_t i) {
+  if constexpr (H) {
+    const float2 v = unpack_h2(reinterpret_cast<const uint32_t*>(s)[i]);
+    return cmake(v.x, v.y);
+  } else {
+    return s[i];
+  }
+}
+template <bool H>
+__device__ __forceinline__ c32 spec_round(c32 v) {
+  if constexpr (H) {
+    const float2 u = unpack_h2(pack_h2(v.x, v.y));
+    return cmake(u.x, u.y);
+  } else {
+    return v;
+  }
+}
+template <bool H>
+__device__ __forceinline__ void spec_store(c32* s, size_t i, c32 v) {
+  if constexpr (H) reinterpret_cast<uint32_t*>(s)[i] = pack_h2(v.x, v.y);
+  else s[i] = v;
+}
+
 template <int BW, int BH>
 struct Shape {
   static constexpr int NROW = BH / 2;  // packed row pairs per block
@@ -269,19 +295,20 @@ __device__ __forceinline__ int xcd_contiguous(int i, int n) {
   return x * q + min(x, r) + i / 8;
 }
 
-template <int BW, int BH>
+template <int BW, int BH, bool H>
 __global__ __launch_bounds__(64) void k_analysis(PlaneArgs P, const void* plane, c32* spec) {
   using S = Shape<BW, BH>;
   __shared__ c32 lds[S::LDS_A];
   const int lane = threadIdx.x, g = lane / S::LPB, j = lane - g * S::LPB;
   const int b = xcd_contiguous(blockIdx.x, gridDim.x) * S::G + g;
   const bool live = b < P.nb;
-  c32* d = spec + (size_t)b * S::BSTR + j;
+  const size_t d = (size_t)b * S::BSTR + j;
   forward_block<BW, BH, S::SPLIT_A>(P, plane, b, live, g, j, lds,
-                        [&](const float(&yr)[BH], const float(&yi)[BH]) {
+                                    [&](const float(&yr)[BH], const float(&yi)[BH]) {
 #pragma unroll
-                          for (int r = 0; r < BH; r++) d[r * S::NCOL] = cmake(yr[r], yi[r]);
-                        });
+                                      for (int r = 0; r < BH; r++)
+                                        spec_store<H>(spec, d + r * S::NCOL, cmake(yr[r], yi[r]));
+                                    });
 }
 
 // ---------------------------------------------------------------- filter
@@ -457,7 +484,7 @@ __device__ __forceinline__ float4 col_factor(const float4* tab, int bh, int col,
 // repacked W'[r] = H0[r] + i Hh[r], H = (F[r] + conj F[-r]) / 2 the Hermitian
 // part that the real inverse transform keeps.
 // Both passes are called by every lane of the block (lb = lane in block).
-template <int BW, int BH, bool SEP, int BM, int DG, int SHP, int PAT>
+template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT>
 __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
@@ -480,7 +507,7 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
     for (int u = 0; u < U; u++) {
       const int r = r0 + (i0 + u) * RS;
 #pragma unroll
-      for (int t = 0; t < 4; t++) va[u][t] = Wn.s[t][base + r * NCOL + c];
+      for (int t = 0; t < 4; t++) va[u][t] = spec_load<H>(Wn.s[t], base + r * NCOL + c);
     }
   };
   auto work = [&](int i0, const c32(&va)[U][4]) {
@@ -514,7 +541,7 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   }
 }
 
-template <int BW, int BH, bool SEP, int BM, int DG, int SHP, int PAT>
+template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT>
 __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
@@ -569,25 +596,25 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
   }
 }
 
-template <int BW, int BH, bool SEP, int BM, int DG, int SHP, int PAT>
+template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT>
 __device__ __forceinline__ void filter_passes(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                               bool live, int lb, c32* Yb, const float4* tab,
                                               float4* kstate, bool kfirst, float dc) {
   if (live)
-    filter_pass1<BW, BH, SEP, BM, DG, SHP, PAT>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+    filter_pass1<BW, BH, SEP, H, BM, DG, SHP, PAT>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
   wave_sync();
   if (live)
-    filter_pass2<BW, BH, SEP, BM, DG, SHP, PAT>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+    filter_pass2<BW, BH, SEP, H, BM, DG, SHP, PAT>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
 }
 
 // The Wiener modes get one instance per (degrid, sharpen, sigma pattern)
 // combination; Kalman and the one-frame modes read the flags at run time.
-template <int BW, int BH, bool SEP, int BM>
+template <int BW, int BH, bool SEP, bool H, int BM>
 __device__ __forceinline__ void filter_mode(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                             bool live, int lb, c32* Yb, const float4* tab,
                                             float4* kstate, bool kfirst, float dc) {
 #define FK_PASSES(DG, SHP, PAT) \
-  filter_passes<BW, BH, SEP, BM, DG, SHP, PAT>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc)
+  filter_passes<BW, BH, SEP, H, BM, DG, SHP, PAT>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc)
   if constexpr (BM >= 2) {
     const int sel = (P.dg ? 4 : 0) | (P.shp ? 2 : 0) | (P.pat ? 1 : 0);
     switch (sel) {
@@ -613,7 +640,7 @@ __device__ __forceinline__ void filter_mode(const PlaneArgs& P, const WindowArgs
 // factors); otherwise from the full tables. When bit `cur` of fwdMask is set,
 // the current frame was transformed by this wave just before and its block DC
 // is read from the last slot of Yb.
-template <int BW, int BH, bool SEP>
+template <int BW, int BH, bool SEP, bool H>
 __device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              bool live, int lb, c32* Yb, const float4* tab,
                                              float4* kstate, bool kfirst, int fwdMask = 0) {
@@ -621,16 +648,17 @@ __device__ __forceinline__ void filter_block(const PlaneArgs& P, const WindowArg
   // block DC X[0][0] = Re W[0] of the current frame, for the degrid term
   float dc = 0.0f;
   if (live && P.dg)
-    dc = ((fwdMask >> Wn.cur) & 1) ? Yb[S::YS - 1].x : Wn.s[Wn.cur][(size_t)b * S::BSTR].x;
+    dc = ((fwdMask >> Wn.cur) & 1) ? Yb[S::YS - 1].x
+                                   : spec_load<H>(Wn.s[Wn.cur], (size_t)b * S::BSTR).x;
   if (fwdMask) wave_sync();  // pass 1 stages over that slot
   // one instance per temporal mode: only the plane's mode is in the hot loop
   switch (P.bm) {
-  case 4: filter_mode<BW, BH, SEP, 4>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  case 3: filter_mode<BW, BH, SEP, 3>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  case 2: filter_mode<BW, BH, SEP, 2>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  case 1: filter_mode<BW, BH, SEP, 1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  case 0: filter_mode<BW, BH, SEP, 0>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
-  default: filter_mode<BW, BH, SEP, -1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc);
+  case 4: filter_mode<BW, BH, SEP, H, 4>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 3: filter_mode<BW, BH, SEP, H, 3>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 2: filter_mode<BW, BH, SEP, H, 2>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 1: filter_mode<BW, BH, SEP, H, 1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  case 0: filter_mode<BW, BH, SEP, H, 0>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc); break;
+  default: filter_mode<BW, BH, SEP, H, -1>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst, dc);
   }
   wave_sync();
 }
@@ -678,7 +706,7 @@ __device__ __forceinline__ void inverse_block(bool live, int j, c32* Yb, float (
 }
 
 // Filter + inverse + synthesis window; writes windowed blocks [b][ly][lx].
-template <int BW, int BH>
+template <int BW, int BH, bool H>
 __global__ __launch_bounds__(64) void k_filter_inverse(PlaneArgs P, WindowArgs Wn,
                                                        float4* kstate, int kfirst,
                                                        float* blocks) {
@@ -688,7 +716,7 @@ __global__ __launch_bounds__(64) void k_filter_inverse(PlaneArgs P, WindowArgs W
   const int b = blockIdx.x * S::G + g;
   const bool live = b < P.nb;
   c32* Yb = lds + g * S::YS;
-  filter_block<BW, BH, false>(P, Wn, b, live, j, Yb, nullptr, kstate, kfirst != 0);
+  filter_block<BW, BH, false, H>(P, Wn, b, live, j, Yb, nullptr, kstate, kfirst != 0);
   float xr[BW], xi[BW];
   inverse_block<BW, BH>(live, j, Yb, xr, xi);
   if (live && j < S::NROW) {

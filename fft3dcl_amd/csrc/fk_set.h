@@ -19,6 +19,10 @@ struct KernelSet {
   FilterFn filter_inverse = nullptr;
   SweepFn sweep = nullptr;
   SweepFn sweepFwd = nullptr;  // with the new frame's analysis inside
+  // the same over a ring of FP16 spectra (square blocks of 16, 32 and 64)
+  AnalysisFn analysisH = nullptr;
+  FilterFn filterH = nullptr;
+  SweepFn sweepH = nullptr;
   int G = 0;
   int sweepThreads = 0;  // 64 * waves per sweep workgroup
   int nbw = 0;           // block columns per sweep strip

@@ -21,10 +21,15 @@ KernelSet make_set() {
   constexpr int LDS_BYTES = 4 * SweepShape<BW, BH, WPW>::LDS_FLOATS;
   constexpr int BY_LDS = 160 * 1024 / LDS_BYTES, BY_WAVES = 8 / WPW;
   KernelSet k;
-  k.analysis = &k_analysis<BW, BH>;
-  k.filter_inverse = &k_filter_inverse<BW, BH>;
-  k.sweep = &k_filter_synth<BW, BH, WPW, false>;
-  k.sweepFwd = &k_filter_synth<BW, BH, WPW, true>;
+  k.analysis = &k_analysis<BW, BH, false>;
+  k.filter_inverse = &k_filter_inverse<BW, BH, false>;
+  k.sweep = &k_filter_synth<BW, BH, WPW, false, false>;
+  k.sweepFwd = &k_filter_synth<BW, BH, WPW, true, false>;
+  if constexpr (BW == BH && BW >= 16) {
+    k.analysisH = &k_analysis<BW, BH, true>;
+    k.filterH = &k_filter_inverse<BW, BH, true>;
+    k.sweepH = &k_filter_synth<BW, BH, WPW, false, true>;
+  }
   k.G = Shape<BW, BH>::G;
   k.sweepThreads = 64 * WPW;
   k.nbw = SweepShape<BW, BH, WPW>::NBW;

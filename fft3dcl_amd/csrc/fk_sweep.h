@@ -107,7 +107,7 @@ __device__ __forceinline__ void store_sample4(const PlaneArgs& P, void* out, siz
   }
 }
 
-template <int BW, int BH, int WPW, bool FWD>
+template <int BW, int BH, int WPW, bool FWD, bool H>
 __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
                                                            float4* kstate, int kfirst,
                                                            SweepArgs A) {
@@ -148,16 +148,18 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
     if (FWD) {
       // analysis of the new frame into the ring; the block DC also goes to
       // the last slot of Yb, which the forward transpose never touches
-      c32* d = A.fwdSpec + (size_t)b * S::BSTR + j;
-      forward_block<BW, BH, S::SPLIT_A>(P, A.fwdPlane, b, colLive, 0, j, Yb,
-                            [&](const float(&yr)[BH], const float(&yi)[BH]) {
+      const size_t d = (size_t)b * S::BSTR + j;
+      forward_block<BW, BH, S::SPLIT_A>(
+          P, A.fwdPlane, b, colLive, 0, j, Yb, [&](const float(&yr)[BH], const float(&yi)[BH]) {
 #pragma unroll
-                              for (int r = 0; r < BH; r++) d[r * S::NCOL] = cmake(yr[r], yi[r]);
-                              if (j == 0) Yb[S::YS - 1] = cmake(yr[0], yi[0]);
-                            });
+            for (int r = 0; r < BH; r++)
+              spec_store<H>(A.fwdSpec, d + r * S::NCOL, cmake(yr[r], yi[r]));
+            // the DC as the filter would read it back from the ring
+            if (j == 0) Yb[S::YS - 1] = spec_round<H>(cmake(yr[0], yi[0]));
+          });
       wave_sync_global();  // the ring stores are read back by the filter
     }
-    filter_block<BW, BH, true>(P, Wn, b, colLive, j, Yb, tab, kstate, kfirst != 0,
+    filter_block<BW, BH, true, H>(P, Wn, b, colLive, j, Yb, tab, kstate, kfirst != 0,
                                FWD ? A.fwdMask : 0);
     float xr[BW], xi[BW];
     inverse_block<BW, BH>(colLive, j, Yb, xr, xi);

@@ -48,9 +48,10 @@ class Instance:
 
     def __init__(self, color_family: int, width: int, height: int, bits: int,
                  sample_type: int, num_frames: int, ssw: int = 1, ssh: int = 1,
-                 lib=None, fuse: bool = False, **kw):
+                 lib=None, fuse: bool = False, half: bool = False, **kw):
         """fuse: let the filter pass transform the last missing window frame
-        itself (FFT3D_OPT_FUSED_ANALYSIS) instead of a separate analysis."""
+        itself (FFT3D_OPT_FUSED_ANALYSIS) instead of a separate analysis.
+        half: store the spectra ring as FP16 (FFT3D_OPT_SPECTRA_FP16)."""
         device = int(kw.pop("device", 2))
         if device == 0:
             raise ValueError("FFT3DCL: device=0 selects the CPU backend, which this HIP "
@@ -72,6 +73,8 @@ class Instance:
                                         chroma=color_family == YUV and q > 0, lib=lib))
         for o in self.objs:
             o.set_option(capi.Plane.OPT_FUSED_ANALYSIS, int(fuse))
+            if half:
+                o.set_option(capi.Plane.OPT_SPECTRA_FP16, 1)
         info = self.objs[0].info
         self.bm, self.bt, self.current = info.bm, info.bt, info.current
 

@@ -163,6 +163,11 @@ int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max);
  *                        transforms the new frame itself (bit-identical,
  *                        one launch and one spectrum read fewer, but slower
  *                        on gfx950 so far: the sweep is latency-bound).
+ *   FFT3D_OPT_SPECTRA_FP16  1: stores the spectra ring as FP16 pairs (round
+ *                        to nearest even; arithmetic stays FP32), half the
+ *                        ring's bytes; drops the ring. Square blocks of 16,
+ *                        32 and 64 only (FFT3D_EUNSUPPORTED otherwise). The
+ *                        reference's precision=0 storage; not exact.
  *   FFT3D_OPT_RESERVE_SPECTRA  allocates that many spectra into the reuse
  *                        pool now (FFT3D_ENOMEM if they do not fit), so that
  *                        a serving loop makes no device allocation later.
@@ -171,7 +176,8 @@ enum {
   FFT3D_OPT_SWEEP = 1,
   FFT3D_OPT_BAND_ROWS = 2,
   FFT3D_OPT_FUSED_ANALYSIS = 3,
-  FFT3D_OPT_RESERVE_SPECTRA = 4
+  FFT3D_OPT_RESERVE_SPECTRA = 4,
+  FFT3D_OPT_SPECTRA_FP16 = 5
 };
 int fft3d_plane_set_option(fft3d_plane* p, int option, int value);
 

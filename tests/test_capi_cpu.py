@@ -288,3 +288,52 @@ def test_emu_forward_process_errors(emu_lib):
     pl.forward_process(3, np.ascontiguousarray(planes[0][3]), [0, 1, 2], out)
     assert pl.has(3)
     pl.close()
+
+
+def _f16_round(x):
+    """Complex array with real and imaginary parts rounded to binary16."""
+    return (x.real.astype(np.float16).astype(np.float32)
+            + 1j * x.imag.astype(np.float16).astype(np.float32)).astype(np.complex64)
+
+
+@pytest.mark.parametrize("kind,kw", [("u8", dict(bt=3)), ("u10", dict(bt=4, sharpen=0.4)),
+                                     ("u8", dict(bt=2, bw=16, bh=16)),
+                                     ("f32", dict(bt=0, bw=64, bh=64))])
+def test_emu_fp16_ring_within_parity(oracle, emu_lib, kind, kw):
+    """FP16 spectra ring (FFT3D_OPT_SPECTRA_FP16): arithmetic stays FP32 and
+    only the stored spectra are rounded, so the output stays within the path's
+    parity tolerance of the oracle (which keeps FP32 spectra)."""
+    bits, st = kind_info(kind)
+    planes = make_clip(17, 120, 88, 5, kind)
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st)
+    got = FFT3DCL(planes, bits=bits, sample_type=st, lib=emu_lib, half=True, **kw)
+    if kind == "f32":
+        assert max_abs(got[0], want[0]) < 1e-3
+    else:
+        assert lsb_diff(got[0], want[0]) <= (2 if bits == 8 else 4)
+
+
+def test_emu_fp16_ring_stores_rounded_spectra(emu_lib):
+    frames = make_clip(18, 96, 72, 1, "f32")[0]
+    full = capi.Plane(capi.make_params(), 96, 72, 32, sample_type=1, lib=emu_lib)
+    half = capi.Plane(capi.make_params(), 96, 72, 32, sample_type=1, lib=emu_lib)
+    half.set_option(capi.Plane.OPT_SPECTRA_FP16, 1)
+    assert half.info.spectrum_bytes * 2 == full.info.spectrum_bytes
+    for pl in (full, half):
+        pl.forward(0, np.ascontiguousarray(frames[0]))
+    a, b = full.read_spectrum(0), half.read_spectrum(0)
+    # columns 1..bw/2-1 are stored bins; columns 0 and bw/2 are unpacked from
+    # a stored pair, so they are only as close as the rounding of that pair
+    assert np.array_equal(b[..., 1:16], _f16_round(a[..., 1:16]))
+    assert np.abs(b - a).max() <= 2.0 ** -10 * np.abs(a).max()
+    full.close()
+    half.close()
+
+
+def test_emu_fp16_ring_option_errors(emu_lib):
+    pl = capi.Plane(capi.make_params(bw=32, bh=16), 96, 72, 8, lib=emu_lib)
+    with pytest.raises(capi.FFT3DError, match="square blocks"):
+        pl.set_option(capi.Plane.OPT_SPECTRA_FP16, 1)
+    with pytest.raises(capi.FFT3DError, match="must be 0 or 1"):
+        pl.set_option(capi.Plane.OPT_SPECTRA_FP16, 3)
+    pl.close()

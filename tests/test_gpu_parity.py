@@ -258,3 +258,22 @@ def test_sweep_equals_unfused_path(gpu, oracle, kind, kw, W, H, R):
         check(kind, out, want[0])
         outs.append(out)
     check(kind, outs[0], outs[1])
+
+
+@pytest.mark.parametrize("name", ["A_640x360_u8_bt3", "B_1080p10_yuv_plane4_bt4", "C_4k10_bt4"])
+def test_fp16_ring_vs_oracle(gpu, oracle, oracle64, name):
+    """FP16 spectra ring (the reference's precision=0 storage, here an
+    option): within the path's tolerance of the FP32 oracle. Prints the max
+    error against the FP32 oracle and its FP64 twin, in LSB."""
+    kind, W, H, n, color, kw, cut = CONFIGS[name]
+    n = min(n, 5)
+    bits, st = kind_info(kind)
+    planes = make_clip(1000 + len(name), W, H, n, kind, color=color, scene_cut=cut)
+    cf = 3 if color == "yuv" else 1
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st, color_family=cf)
+    want64, _ = oracle64.run_clip(script(**kw), planes, bits, st, color_family=cf)
+    got = FFT3DCL(planes, color_family=cf, bits=bits, sample_type=st, lib=gpu, half=True, **kw)
+    for g, w, w64 in zip(got, want, want64):
+        d = check(kind, g, w)
+        print(f"{name} FP16 ring: max |GPU - oracle| {d} LSB, "
+              f"max |GPU - FP64 oracle| {lsb_diff(g, w64)} LSB")
