@@ -23,8 +23,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops"]
-# translation units: the C ABI, and the kernels of each block width
-BLOCK_WIDTHS = (4, 8, 16, 32, 64)
+# translation units: the C ABI, and the kernels of each block shape
+BLOCK_SIZES = (4, 8, 16, 32, 64)
+JOBS = min(8, os.cpu_count() or 1)
 
 
 def _run(cmd: list[str]) -> None:
@@ -35,25 +36,30 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_hip(out: Path = HIP_LIB) -> Path:
-    """Compiles the C ABI unit and one kernel unit per block width in
-    parallel, then links them into libfft3d_hip.so."""
+    """Compiles the C ABI unit and one kernel unit per block shape, JOBS at a
+    time, then links them into libfft3d_hip.so."""
     import tempfile
+    from concurrent.futures import ThreadPoolExecutor
     inc = [f"-I{INCLUDE}", f"-I{CSRC}"]
     with tempfile.TemporaryDirectory(prefix="fft3d_build_") as tmpdir:
         units = [(CSRC / "fft3d_capi.hip", [], Path(tmpdir) / "capi.o")]
-        units += [(CSRC / "fk_set_bw.hip", [f"-DFK_BW={bw}"], Path(tmpdir) / f"bw{bw}.o")
-                  for bw in BLOCK_WIDTHS]
-        procs = [(subprocess.Popen([HIPCC, *HIP_FLAGS, *inc, *d, "-c", str(src), "-o", str(o)],
-                                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True),
-                  src) for src, d, o in units]
-        failed = []
-        for p, src in procs:
-            log, _ = p.communicate()
-            if p.returncode != 0:
-                failed.append(f"{src.name}:\n{log}")
+        units += [(CSRC / "fk_set_bw.hip", [f"-DFK_BW={bw}", f"-DFK_BH={bh}"],
+                   Path(tmpdir) / f"s{bw}x{bh}.o")
+                  for bw in BLOCK_SIZES for bh in BLOCK_SIZES]
+        # largest shapes first: they take longest
+        units.sort(key=lambda u: -sum(int(d.split("=")[1]) for d in u[1]))
+
+        def compile_unit(u):
+            src, d, o = u
+            r = subprocess.run([HIPCC, *HIP_FLAGS, *inc, *d, "-c", str(src), "-o", str(o)],
+                               capture_output=True, text=True)
+            return None if r.returncode == 0 else f"{src.name} {' '.join(d)}:\n{r.stdout}{r.stderr}"
+
+        with ThreadPoolExecutor(JOBS) as ex:
+            failed = [f for f in ex.map(compile_unit, units) if f]
         if failed:
             sys.stderr.write("\n".join(failed))
-            raise RuntimeError("build failed: " + ", ".join(f.split(":")[0] for f in failed))
+            raise RuntimeError("build failed: " + "; ".join(f.split(":")[0] for f in failed))
         tmp = out.with_suffix(".so.tmp")
         _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC",
               *[str(o) for _, _, o in units], "-o", str(tmp)])

@@ -31,13 +31,25 @@ using fk::KernelSet;
 using fk::SweepFn;
 using fk::SynthFn;
 
+template <int BW>
+KernelSet kernels_for_bh(int bh) {
+  switch (bh) {
+  case 4: return fk::kernel_set<BW, 4>();
+  case 8: return fk::kernel_set<BW, 8>();
+  case 16: return fk::kernel_set<BW, 16>();
+  case 32: return fk::kernel_set<BW, 32>();
+  case 64: return fk::kernel_set<BW, 64>();
+  }
+  return KernelSet{};
+}
+
 KernelSet kernels_for(int bw, int bh) {
   switch (bw) {
-  case 4: return fk::kernel_set<4>(bh);
-  case 8: return fk::kernel_set<8>(bh);
-  case 16: return fk::kernel_set<16>(bh);
-  case 32: return fk::kernel_set<32>(bh);
-  case 64: return fk::kernel_set<64>(bh);
+  case 4: return kernels_for_bh<4>(bh);
+  case 8: return kernels_for_bh<8>(bh);
+  case 16: return kernels_for_bh<16>(bh);
+  case 32: return kernels_for_bh<32>(bh);
+  case 64: return kernels_for_bh<64>(bh);
   }
   return KernelSet{};
 }

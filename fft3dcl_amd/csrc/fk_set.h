@@ -1,6 +1,6 @@
 // Kernel entry points of one block shape, as launched by the C ABI
-// (fft3d_capi.hip). The templates are instantiated per block width in
-// fk_set_bw.hip, one translation unit per width, so that they compile in
+// (fft3d_capi.hip). The templates are instantiated per block shape in
+// fk_set_bw.hip, one translation unit per shape, so that they compile in
 // parallel.
 #pragma once
 
@@ -29,9 +29,9 @@ struct KernelSet {
   int sweepPerCU = 0;    // sweep workgroups one CU holds (LDS, 2 waves per SIMD)
 };
 
-// The kernels of block shape (BW, bh); an empty set for an unsupported bh.
-// Defined for BW in {4, 8, 16, 32, 64} by fk_set_bw.hip.
-template <int BW>
-KernelSet kernel_set(int bh);
+// The kernels of block shape (BW, BH), for BW, BH in {4, 8, 16, 32, 64};
+// each shape is defined by its own compile of fk_set_bw.hip.
+template <int BW, int BH>
+KernelSet kernel_set();
 
 }  // namespace fk

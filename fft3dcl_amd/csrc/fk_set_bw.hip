@@ -1,9 +1,9 @@
-// The kernel set of every block height for one block width FK_BW (compiled
-// once per width: -DFK_BW=4, 8, 16, 32 or 64).
+// The kernel set of one block shape FK_BW x FK_BH (compiled once per shape:
+// -DFK_BW=... -DFK_BH=..., each in 4, 8, 16, 32, 64).
 #include "fk_set.h"
 
-#ifndef FK_BW
-#error "FK_BW (block width) must be defined"
+#if !defined(FK_BW) || !defined(FK_BH)
+#error "FK_BW and FK_BH (block width and height) must be defined"
 #endif
 
 namespace fk {
@@ -39,18 +39,11 @@ KernelSet make_set() {
 
 }  // namespace
 
-template <int BW>
-KernelSet kernel_set(int bh) {
-  switch (bh) {
-  case 4: return make_set<BW, 4>();
-  case 8: return make_set<BW, 8>();
-  case 16: return make_set<BW, 16>();
-  case 32: return make_set<BW, 32>();
-  case 64: return make_set<BW, 64>();
-  }
-  return KernelSet{};
+template <int BW, int BH>
+KernelSet kernel_set() {
+  return make_set<BW, BH>();
 }
 
-template KernelSet kernel_set<FK_BW>(int bh);
+template KernelSet kernel_set<FK_BW, FK_BH>();
 
 }  // namespace fk

@@ -1,3 +1,3 @@
-// One block width's kernel unit (-DFK_BW=...), compiled as host C++ against
-// the HIP emulation.
+// One block shape's kernel unit (-DFK_BW=... -DFK_BH=...), compiled as host
+// C++ against the HIP emulation.
 #include "fk_set_bw.hip"
