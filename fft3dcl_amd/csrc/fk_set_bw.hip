@@ -13,11 +13,7 @@ namespace {
 // would leave room for only one workgroup per CU.
 template <int BW, int BH>
 KernelSet make_set() {
-#ifdef FK_SWEEP_WAVES
-  constexpr int WPW = FK_SWEEP_WAVES;
-#else
   constexpr int WPW = (BW * BH >= 4096) ? 2 : 4;
-#endif
   constexpr int LDS_BYTES = 4 * SweepShape<BW, BH, WPW>::LDS_FLOATS;
   constexpr int BY_LDS = 160 * 1024 / LDS_BYTES, BY_WAVES = 8 / WPW;
   KernelSet k;
