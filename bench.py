@@ -84,16 +84,20 @@ def host_buffer_rates(frames, steps=24):
     host buffers (the VapourSynth plugin's boundary): PCIe included, both
     directions, as the C ABI does it (host-to-device copy before the analysis,
     device-to-host copy after the synthesis, the call returns when the host
-    side is done). Pageable numpy buffers, then pinned ones."""
+    side is done). Pageable numpy buffers through the library's pinned chunks
+    (its default), the same buffers handed to hipMemcpyAsync as they are, and
+    pinned buffers."""
     import numpy as np
     import torch
     from fft3dcl_amd.filter import GRAY, Instance
     res = {}
     pinned = torch.from_numpy(frames).pin_memory()
     out_pinned = torch.empty((H, W), dtype=torch.int16).pin_memory()
-    for kind in ("pageable", "pinned"):
+    for kind in ("pageable", "pageable_unstaged", "pinned"):
         inst = Instance(GRAY, W, H, BITS, 0, steps + 16, **PARAMS)
-        if kind == "pageable":
+        if kind == "pageable_unstaged":
+            inst.objs[0].set_option(inst.objs[0].OPT_HOST_STAGING, 0)
+        if kind.startswith("pageable"):
             out = np.zeros((H, W), np.uint16)
             srcs = [lambda fn: frames[fn % NUNIQ]]
             dst = out

@@ -226,6 +226,7 @@ class Plane:
 
     OPT_SWEEP, OPT_BAND_ROWS, OPT_FUSED_ANALYSIS, OPT_RESERVE_SPECTRA = 1, 2, 3, 4
     OPT_SPECTRA_FP16 = 5
+    OPT_HOST_STAGING, OPT_STAGING_CHUNK = 6, 7
 
     def set_option(self, option: int, value: int):
         self._check(self.lib.fft3d_plane_set_option(self.h, option, int(value)))

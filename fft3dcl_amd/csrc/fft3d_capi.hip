@@ -9,6 +9,7 @@
 #include "fk_set.h"
 #include "fft3d_params.h"
 #include "fft3d_hip.h"
+#include "fk_xfer.h"
 
 #include <algorithm>
 #include <atomic>
@@ -123,6 +124,8 @@ struct fft3d_plane {
   float* dTables = nullptr;  // windows, pattern, sharpen, degrid
   void* dStage = nullptr;    // host-upload / host-readback staging (W*H*bps)
   void* dStageIn = nullptr;  // host upload of a fused analysis (W*H*bps, on first use)
+  fx::Staging stg;           // pinned chunks for pageable host buffers (on first use)
+  bool stagingOn = true;     // option
   float* dBlocks = nullptr;  // windowed blocks awaiting overlap-add (unfused path)
   float4* dKalman = nullptr; // bm == 0 state
   bool kalmanFirst = true;
@@ -190,6 +193,7 @@ struct fft3d_plane {
     (void)hipFree(dTables);
     (void)hipFree(dStage);
     (void)hipFree(dStageIn);
+    stg.release();
     (void)hipFree(dBlocks);
     (void)hipFree(dKalman);
     (void)hipFree(dPart);
@@ -210,6 +214,28 @@ struct fft3d_plane {
   }
 
   hipStream_t stream(void* s) const { return s ? static_cast<hipStream_t>(s) : own; }
+
+  // Host side of the plane transfers: pageable buffers through the pinned
+  // chunks (fk_xfer.h), others straight to the DMA engine. If the chunks
+  // cannot be allocated the copy goes to hipMemcpyAsync as it is.
+  bool staged(const void* host) {
+    if (!stagingOn || !fx::pageable(host)) return false;
+    if (!stg.ready() && stg.init() != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    return true;
+  }
+  hipError_t upload(void* dDst, const void* src, size_t n, hipStream_t s) {
+    if (staged(src)) return stg.upload(dDst, src, n, s);
+    return hipMemcpyAsync(dDst, src, n, hipMemcpyHostToDevice, s);
+  }
+  // returns once the plane is in dst
+  hipError_t download(void* dst, const void* dSrc, size_t n, hipStream_t s) {
+    if (staged(dst)) return stg.download(dst, dSrc, n, s);
+    const hipError_t e = hipMemcpyAsync(dst, dSrc, n, hipMemcpyDeviceToHost, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+  }
 
   fk::c32* spectrum(int fn) {
     auto it = ring.find(fn);
@@ -380,12 +406,9 @@ int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipS
     prof_end(t0, K_SYNTH, s);
     if (!launched("synthesize")) return FFT3D_ELAUNCH;
   }
-  if (!dst_on_device) {
-    if (hipMemcpyAsync(dst, dStage, need, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(FFT3D_ELAUNCH, "readback failed");
-    }
+  if (!dst_on_device && download(dst, dStage, need, s) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(FFT3D_ELAUNCH, "readback failed");
   }
   return FFT3D_OK;
 }
@@ -612,7 +635,7 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
   if (!spec) return p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed");
   const void* plane = src;
   if (!src_on_device) {
-    if (hipMemcpyAsync(p->dStage, src, need, hipMemcpyHostToDevice, s) != hipSuccess) {
+    if (p->upload(p->dStage, src, need, s) != hipSuccess) {
       (void)hipGetLastError();
       p->ring.erase(fn);
       p->retire(spec);
@@ -667,7 +690,7 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
   if (!src_on_device) {
     if (!p->dStageIn && !p->alloc(&p->dStageIn, need))
       return p->fail(FFT3D_ENOMEM, "GPU alloc failed (upload staging)");
-    if (hipMemcpyAsync(p->dStageIn, src, need, hipMemcpyHostToDevice, s) != hipSuccess) {
+    if (p->upload(p->dStageIn, src, need, s) != hipSuccess) {
       (void)hipGetLastError();
       return p->fail(FFT3D_ELAUNCH, "plane upload failed");
     }
@@ -749,6 +772,25 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
       return p->fail(FFT3D_EINVAL, "fused analysis option must be 0 or 1");
     p->fuseOn = value == 1;
     return FFT3D_OK;
+  case FFT3D_OPT_HOST_STAGING:
+    if (value != 0 && value != 1)
+      return p->fail(FFT3D_EINVAL, "host staging option must be 0 or 1");
+    p->stagingOn = value == 1;
+    return FFT3D_OK;
+  case FFT3D_OPT_STAGING_CHUNK: {
+    if (value != 0 && (value < 256 || value > (256 << 20) || value % 64 != 0))
+      return p->fail(FFT3D_EINVAL, "staging chunk must be 0 or a multiple of 64 in 256..256 MiB");
+    const size_t chunk = value ? (size_t)value : fx::Staging::kDefaultChunk;
+    if (chunk == p->stg.chunk) return FFT3D_OK;
+    // queued DMAs may still use the old chunks
+    if (p->stg.ready() && hipDeviceSynchronize() != hipSuccess) {
+      (void)hipGetLastError();
+      return p->fail(FFT3D_ELAUNCH, "synchronize failed");
+    }
+    p->stg.release();
+    p->stg.chunk = chunk;
+    return FFT3D_OK;
+  }
   }
   return p->fail(FFT3D_EINVAL, "unknown option");
 }

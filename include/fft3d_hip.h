@@ -171,13 +171,22 @@ int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max);
  *   FFT3D_OPT_RESERVE_SPECTRA  allocates that many spectra into the reuse
  *                        pool now (FFT3D_ENOMEM if they do not fit), so that
  *                        a serving loop makes no device allocation later.
+ *   FFT3D_OPT_HOST_STAGING  1 (default): host buffers in pageable memory go
+ *                        through the plane's pinned chunks, copied by host
+ *                        threads while the DMA engine moves the neighbouring
+ *                        chunks; 0: handed to hipMemcpyAsync as they are.
+ *                        Page-locked and device buffers never use the chunks.
+ *   FFT3D_OPT_STAGING_CHUNK  bytes per pinned chunk (0 = 4 MiB; else a
+ *                        multiple of 64 from 256 to 256 MiB).
  * Returns FFT3D_EINVAL for an unknown option or value. */
 enum {
   FFT3D_OPT_SWEEP = 1,
   FFT3D_OPT_BAND_ROWS = 2,
   FFT3D_OPT_FUSED_ANALYSIS = 3,
   FFT3D_OPT_RESERVE_SPECTRA = 4,
-  FFT3D_OPT_SPECTRA_FP16 = 5
+  FFT3D_OPT_SPECTRA_FP16 = 5,
+  FFT3D_OPT_HOST_STAGING = 6,
+  FFT3D_OPT_STAGING_CHUNK = 7
 };
 int fft3d_plane_set_option(fft3d_plane* p, int option, int value);
 

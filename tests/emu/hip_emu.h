@@ -208,6 +208,30 @@ inline hipError_t hipEventCreate(hipEvent_t* e) {
   *e = nullptr;
   return hipSuccess;
 }
+#define hipEventDisableTiming 2
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+// host memory: every buffer of the emulation is plain host memory; the
+// pointer query reports all of them as pageable (the staged transfer path)
+#define hipHostMallocDefault 0
+inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
+  if (emu_fail_alloc_) return hipErrorOutOfMemory;
+  *p = std::calloc(1, n ? n : 1);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipHostFree(void* p) {
+  std::free(p);
+  return hipSuccess;
+}
+enum hipMemoryType { hipMemoryTypeUnregistered = 0, hipMemoryTypeHost = 1, hipMemoryTypeDevice = 2 };
+struct hipPointerAttribute_t {
+  hipMemoryType type;
+  int device;
+};
+inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t* a, const void*) {
+  a->type = hipMemoryTypeUnregistered;
+  a->device = 0;
+  return hipSuccess;
+}
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }

@@ -267,6 +267,41 @@ def test_emu_forward_process_equals_forward_then_process(emu_lib, kind, kw, W, H
         assert np.array_equal(res[0][1][fn], res[1][1][fn])
 
 
+@pytest.mark.parametrize("chunk,fuse", [(256, False), (4160, True), (0, False)])
+def test_emu_staged_host_transfers(emu_lib, chunk, fuse):
+    """Pageable host buffers through the pinned chunks (many chunks with a
+    ragged last one, or one chunk split over the copy threads: the plane is
+    above 256 KiB) give the same frames as direct copies."""
+    from fft3dcl_amd.filter import Instance
+    W, H = 400, 330  # 264,000 bytes of u16
+    planes = make_clip(16, W, H, 3, "u10")
+    outs = []
+    for staging, ch in ((0, 0), (1, chunk)):
+        inst = Instance(1, W, H, 10, 0, 3, lib=emu_lib, fuse=fuse, bt=3, sharpen=0.3)
+        pl = inst.objs[0]
+        pl.set_option(capi.Plane.OPT_HOST_STAGING, staging)
+        pl.set_option(capi.Plane.OPT_STAGING_CHUNK, ch)
+        out = np.zeros_like(planes[0])
+        for n in range(3):
+            inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
+        inst.close()
+        outs.append(out)
+    assert outs[0].any()
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_emu_staging_options(emu_lib):
+    pl = capi.Plane(capi.make_params(bt=3), 96, 64, 8, lib=emu_lib)
+    for opt, bad in ((capi.Plane.OPT_HOST_STAGING, 2), (capi.Plane.OPT_STAGING_CHUNK, 300),
+                     (capi.Plane.OPT_STAGING_CHUNK, 128), (capi.Plane.OPT_STAGING_CHUNK, -64)):
+        with pytest.raises(capi.FFT3DError) as e:
+            pl.set_option(opt, bad)
+        assert e.value.code == -1
+    pl.set_option(capi.Plane.OPT_STAGING_CHUNK, 1 << 20)
+    pl.set_option(capi.Plane.OPT_STAGING_CHUNK, 0)
+    pl.close()
+
+
 def test_emu_forward_process_errors(emu_lib):
     planes = make_clip(15, 96, 64, 4, "u8")
     pl = capi.Plane(capi.make_params(bt=3), 96, 64, 8, lib=emu_lib)

@@ -1,7 +1,9 @@
 """Runs a set of emulated kernel launches (tests/emu, thread-per-lane CPU
 emulation of the HIP kernels) under AddressSanitizer in a child process, so an
 out-of-bounds global or LDS access in the kernels' index arithmetic fails on
-the CPU before it could fault a GPU. LDS arrays are function statics in the
+the CPU before it could fault a GPU. The host buffers go through the pinned
+chunks of the staged transfers (ragged chunks, and the threaded copy of a
+plane above 256 KiB). LDS arrays are function statics in the
 emulation, so their overruns are caught as global-buffer overflows."""
 import os
 import subprocess
@@ -30,10 +32,18 @@ for kind, bits, st, kw in cases:
         for sweep in (1, 0):
             inst = Instance(1, W, H, bits, st, 3, lib=lib, **kw)
             inst.objs[0].set_option(1, sweep)
+            inst.objs[0].set_option(7, 320 if sweep else 0)  # pinned chunks of 320 bytes
             out = np.zeros_like(planes[0])
             for n in range(3):
                 inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
             inst.close()
+# a plane above 256 KiB: one chunk copied by several host threads
+planes = make_clip(2, 300, 220, 2, "f32")
+inst = Instance(1, 300, 220, 32, 1, 2, lib=lib, bt=2)
+out = np.zeros_like(planes[0])
+for n in range(2):
+    inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
+inst.close()
 print("asan-ok")
 """
 

@@ -135,6 +135,33 @@ def test_device_resident_path_equals_host_path(gpu):
     assert np.array_equal(dev_out.cpu().numpy().view(np.uint16), host)
 
 
+@pytest.mark.parametrize("chunk,fuse", [(0, False), (1 << 20, False), (4160, True)])
+def test_host_transfers_staged_direct_pinned(gpu, chunk, fuse):
+    """The same frames whichever way host buffers travel: pageable numpy
+    buffers through the pinned chunks (4 MiB, 1 MiB, or 4160-byte chunks with
+    a ragged last one), pageable buffers handed to hipMemcpyAsync as they
+    are, and page-locked buffers straight to the DMA engine."""
+    W, H, n = 1920, 1080, 5
+    frames = make_plane_sequence(11, W, H, n, "u10")
+
+    def run(staging, src, out):
+        inst = Instance(1, W, H, 10, 0, n, bt=4, sharpen=0.4, lib=gpu, fuse=fuse)
+        inst.objs[0].set_option(capi.Plane.OPT_HOST_STAGING, staging)
+        inst.objs[0].set_option(capi.Plane.OPT_STAGING_CHUNK, chunk)
+        for k in range(n):
+            inst.get_frame(k, [lambda fn: src[fn]], [out[k]])
+        inst.close()
+        return out
+
+    staged = run(1, frames, np.zeros_like(frames))
+    direct = run(0, frames, np.zeros_like(frames))
+    pinned_in = torch.from_numpy(frames).pin_memory()
+    pinned_out = run(1, pinned_in, torch.zeros((n, H, W), dtype=torch.int16).pin_memory())
+    assert direct.any()
+    assert np.array_equal(staged, direct)
+    assert np.array_equal(pinned_out.numpy().view(np.uint16), direct)
+
+
 def test_spectrum_vs_oracle(gpu, oracle):
     from oracle_lib import Plane as OPlane
     frames = make_plane_sequence(11, 333, 201, 1, "f32")
