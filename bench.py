@@ -86,15 +86,18 @@ def host_buffer_rates(frames, steps=24):
     device-to-host copy after the synthesis, the call returns when the host
     side is done). Pageable numpy buffers through the library's pinned chunks
     (its default), the same buffers handed to hipMemcpyAsync as they are, and
-    pinned buffers."""
+    pinned buffers; then pageable and pinned buffers with each request
+    prefetching the next one's new frame (its upload overlaps the request)."""
     import numpy as np
     import torch
     from fft3dcl_amd.filter import GRAY, Instance
     res = {}
     pinned = torch.from_numpy(frames).pin_memory()
     out_pinned = torch.empty((H, W), dtype=torch.int16).pin_memory()
-    for kind in ("pageable", "pageable_unstaged", "pinned"):
-        inst = Instance(GRAY, W, H, BITS, 0, steps + 16, **PARAMS)
+    for kind in ("pageable", "pageable_unstaged", "pinned", "pageable_prefetch",
+                 "pinned_prefetch"):
+        inst = Instance(GRAY, W, H, BITS, 0, steps + 16, prefetch=kind.endswith("prefetch"),
+                        **PARAMS)
         if kind == "pageable_unstaged":
             inst.objs[0].set_option(inst.objs[0].OPT_HOST_STAGING, 0)
         if kind.startswith("pageable"):

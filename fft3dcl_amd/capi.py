@@ -58,7 +58,7 @@ EXPORTS = [
     "fft3d_plane_error", "fft3d_plane_has", "fft3d_plane_forward", "fft3d_plane_process",
     "fft3d_plane_prune", "fft3d_plane_clear", "fft3d_live_instances", "fft3d_live_bytes",
     "fft3d_plane_get_info", "fft3d_plane_get_table", "fft3d_plane_read_spectrum",
-    "fft3d_plane_forward_process",
+    "fft3d_plane_forward_process", "fft3d_plane_prefetch",
 ]
 
 
@@ -97,6 +97,7 @@ def load(path: str | Path | None = None) -> C.CDLL:
     L.fft3d_plane_process.argtypes = [vp, C.POINTER(C.c_int), i, vp, sz, i, vp]
     L.fft3d_plane_forward_process.argtypes = [vp, i, vp, sz, i, C.POINTER(C.c_int), i, vp, sz,
                                               i, vp]
+    L.fft3d_plane_prefetch.argtypes = [vp, i, vp, sz]
     L.fft3d_plane_prune.argtypes = [vp, i, i]
     L.fft3d_plane_prune.restype = None
     L.fft3d_plane_clear.argtypes = [vp]
@@ -198,17 +199,32 @@ class Plane:
 
     def process(self, window, dst, on_device: bool = False, stream: int | None = None):
         w = (C.c_int * len(window))(*window)
-        self._check(self.lib.fft3d_plane_process(self.h, w, len(window), _ptr(dst),
-                                                 self.nbytes, int(on_device), stream))
+        try:
+            self._check(self.lib.fft3d_plane_process(self.h, w, len(window), _ptr(dst),
+                                                     self.nbytes, int(on_device), stream))
+        finally:
+            self._pending = None
 
     def forward_process(self, fn: int, src, window, dst, src_on_device: bool = False,
                         dst_on_device: bool = False, stream: int | None = None):
         """forward(fn, src) then process(window, dst) in one call; with
         OPT_FUSED_ANALYSIS set, frame fn is transformed inside the sweep."""
         w = (C.c_int * len(window))(*window)
-        self._check(self.lib.fft3d_plane_forward_process(
-            self.h, fn, _ptr(src), self.nbytes, int(src_on_device), w, len(window), _ptr(dst),
-            self.nbytes, int(dst_on_device), stream))
+        try:
+            self._check(self.lib.fft3d_plane_forward_process(
+                self.h, fn, _ptr(src), self.nbytes, int(src_on_device), w, len(window),
+                _ptr(dst), self.nbytes, int(dst_on_device), stream))
+        finally:
+            self._pending = None
+
+    def prefetch(self, fn: int, src):
+        """Requests host plane fn in the ring; the next process or
+        forward_process call carries it out alongside its own work (host
+        buffers only). src is kept alive until that call returns."""
+        if getattr(src, "is_cuda", False):
+            raise ValueError("prefetch takes host buffers")
+        self._check(self.lib.fft3d_plane_prefetch(self.h, fn, _ptr(src), self.nbytes))
+        self._pending = src
 
     def prune(self, lo: int, hi: int):
         self.lib.fft3d_plane_prune(self.h, lo, hi)

@@ -126,6 +126,18 @@ struct fft3d_plane {
   void* dStageIn = nullptr;  // host upload of a fused analysis (W*H*bps, on first use)
   fx::Staging stg;           // pinned chunks for pageable host buffers (on first use)
   bool stagingOn = true;     // option
+
+  // fft3d_plane_prefetch: its own stream, upload buffers, and the events that
+  // order it against the plane's other calls (all created on first use)
+  hipStream_t up = nullptr;
+  void* dStageUp = nullptr;
+  fx::Staging stgUp;
+  hipEvent_t lastCall = nullptr;      // end of the latest other call
+  hipEvent_t upDone = nullptr;        // end of the latest prefetch
+  std::map<int, hipEvent_t> upReady;  // prefetched spectrum -> end of its analysis
+  std::vector<hipEvent_t> evSpare;
+  int pendFn = -1;                    // requested prefetch, run by the next process call
+  const void* pendSrc = nullptr;
   float* dBlocks = nullptr;  // windowed blocks awaiting overlap-add (unfused path)
   float4* dKalman = nullptr; // bm == 0 state
   bool kalmanFirst = true;
@@ -194,6 +206,13 @@ struct fft3d_plane {
     (void)hipFree(dStage);
     (void)hipFree(dStageIn);
     stg.release();
+    (void)hipFree(dStageUp);
+    stgUp.release();
+    for (auto& kv : upReady) (void)hipEventDestroy(kv.second);
+    for (auto e : evSpare) (void)hipEventDestroy(e);
+    if (lastCall) (void)hipEventDestroy(lastCall);
+    if (upDone) (void)hipEventDestroy(upDone);
+    if (up) (void)hipStreamDestroy(up);
     (void)hipFree(dBlocks);
     (void)hipFree(dKalman);
     (void)hipFree(dPart);
@@ -233,15 +252,30 @@ struct fft3d_plane {
   // returns once the plane is in dst
   hipError_t download(void* dst, const void* dSrc, size_t n, hipStream_t s) {
     if (staged(dst)) return stg.download(dst, dSrc, n, s);
-    const hipError_t e = hipMemcpyAsync(dst, dSrc, n, hipMemcpyDeviceToHost, s);
+    const hipError_t e = stagingOn && !fx::pageable(dst)
+                             ? fx::copy_to_host(dst, dSrc, n, s)
+                             : hipMemcpyAsync(dst, dSrc, n, hipMemcpyDeviceToHost, s);
     return e != hipSuccess ? e : hipStreamSynchronize(s);
   }
 
-  fk::c32* spectrum(int fn) {
+  // The ring entry of frame fn, taken from the pool or allocated if new, for
+  // a call that writes it on stream s. Once prefetch is in use, s first waits
+  // for a prefetch still writing that entry, or for the latest prefetch when
+  // the buffer comes from the pool (it may have held a pruned prefetched
+  // frame). A prefetch itself (s == up) is ordered by its own wait.
+  fk::c32* spectrum(int fn, hipStream_t s) {
     auto it = ring.find(fn);
-    if (it != ring.end()) return it->second;
+    if (it != ring.end()) {
+      auto r = upReady.find(fn);
+      if (r != upReady.end() && s != up) {
+        if (hipStreamWaitEvent(s, r->second, 0) != hipSuccess) return nullptr;
+        forget(fn);
+      }
+      return it->second;
+    }
     fk::c32* m = nullptr;
     if (!pool.empty()) {
+      if (up && s != up && hipStreamWaitEvent(s, upDone, 0) != hipSuccess) return nullptr;
       m = pool.back();
       pool.pop_back();
     } else if (!alloc(reinterpret_cast<void**>(&m), specBytes)) {
@@ -249,6 +283,30 @@ struct fft3d_plane {
     }
     ring[fn] = m;
     return m;
+  }
+
+  // drops the prefetch event of frame fn (its ring entry is going away)
+  void forget(int fn) {
+    auto r = upReady.find(fn);
+    if (r == upReady.end()) return;
+    evSpare.push_back(r->second);
+    upReady.erase(r);
+  }
+  void forget_all() {
+    for (auto& kv : upReady) evSpare.push_back(kv.second);
+    upReady.clear();
+  }
+  // s waits for the prefetches of the window's frames
+  bool wait_prefetched(const int* win, int nwin, hipStream_t s) {
+    for (int i = 0; i < nwin; i++) {
+      auto r = upReady.find(win[i]);
+      if (r != upReady.end() && hipStreamWaitEvent(s, r->second, 0) != hipSuccess) return false;
+    }
+    return true;
+  }
+  // marks the end of a call on stream s, for the next prefetch to wait on
+  void end_call(hipStream_t s) {
+    if (up) (void)hipEventRecord(lastCall, s);
   }
 
   void release(fk::c32* m) {
@@ -352,6 +410,7 @@ struct fft3d_plane {
   // mask.
   int run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipStream_t s,
           const void* fwdPlane, fk::c32* fwdSpec, int fwdMask);
+  void run_prefetch();
 
   bool launched(const char* what) {
     hipError_t e = hipGetLastError();
@@ -406,11 +465,89 @@ int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipS
     prof_end(t0, K_SYNTH, s);
     if (!launched("synthesize")) return FFT3D_ELAUNCH;
   }
+  if (pendFn >= 0) run_prefetch();
   if (!dst_on_device && download(dst, dStage, need, s) != hipSuccess) {
     (void)hipGetLastError();
     return fail(FFT3D_ELAUNCH, "readback failed");
   }
   return FFT3D_OK;
+}
+
+// The pending prefetch: frame pendFn from host plane pendSrc into the ring on
+// the upload stream. Runs inside a process call, after its launches, so that
+// the host copies overlap that call's kernels and the upload overlaps its
+// readback. A failure only drops the request (the frame is then forwarded
+// when a window first needs it).
+void fft3d_plane::run_prefetch() {
+  const int fn = pendFn;
+  const void* src = pendSrc;
+  pendFn = -1;
+  pendSrc = nullptr;
+  if (fn < 0 || ring.count(fn)) return;
+  const size_t need = (size_t)d.width * d.height * bps;
+  if (!up) {
+    // first use: everything queued so far on any stream comes first
+    bool ok = hipDeviceSynchronize() == hipSuccess &&
+              hipStreamCreateWithFlags(&up, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&lastCall, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&upDone, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventRecord(lastCall, up) == hipSuccess &&
+         hipEventRecord(upDone, up) == hipSuccess;
+    if (!ok) {
+      (void)hipGetLastError();
+      return;
+    }
+  }
+  // the plane's earlier calls may still read a pooled spectrum
+  if (hipStreamWaitEvent(up, lastCall, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  if (!dStageUp && !alloc(&dStageUp, need)) return;
+  fk::c32* spec = spectrum(fn, up);
+  if (!spec) {
+    (void)hipGetLastError();
+    return;
+  }
+  auto drop = [&]() {
+    (void)hipGetLastError();
+    ring.erase(fn);
+    retire(spec);
+  };
+  bool staged = stagingOn && fx::pageable(src);
+  if (staged && !stgUp.ready()) {
+    // pinned chunks for the whole plane: the host copies never wait for a DMA
+    stgUp.chunk = stg.chunk;
+    const size_t n = (need + stgUp.chunk - 1) / stgUp.chunk;
+    if (stgUp.init((int)std::min<size_t>(n, fx::Staging::MAXBUF)) != hipSuccess) {
+      (void)hipGetLastError();
+      staged = false;
+    }
+  }
+  const hipError_t e = staged ? stgUp.upload(dStageUp, src, need, up)
+                              : hipMemcpyAsync(dStageUp, src, need, hipMemcpyHostToDevice, up);
+  if (e != hipSuccess) return drop();
+  const int grid = (nb + ks.G - 1) / ks.G;
+  AnalysisFn k = half ? ks.analysisH : ks.analysis;
+  hipEvent_t t0 = prof_begin(up);
+  FK_LAUNCH(k, dim3(grid), dim3(64), up, args, dStageUp, spec);
+  prof_end(t0, K_ANALYSIS, up);
+  if (hipGetLastError() != hipSuccess) return drop();
+  hipEvent_t ev = nullptr;
+  if (!evSpare.empty()) {
+    ev = evSpare.back();
+    evSpare.pop_back();
+  } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+    ev = nullptr;
+  }
+  if (!ev || hipEventRecord(ev, up) != hipSuccess || hipEventRecord(upDone, up) != hipSuccess) {
+    (void)hipGetLastError();
+    if (ev) evSpare.push_back(ev);
+    // no event to order the readers by: finish it here instead
+    if (hipStreamSynchronize(up) != hipSuccess) drop();
+    return;
+  }
+  upReady[fn] = ev;
 }
 
 extern "C" {
@@ -631,7 +768,7 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
   const size_t need = (size_t)p->d.width * p->d.height * p->bps;
   if (bytes < need) return p->fail(FFT3D_EINVAL, "plane buffer too small");
   hipStream_t s = p->stream(stream);
-  fk::c32* spec = p->spectrum(fn);
+  fk::c32* spec = p->spectrum(fn, s);
   if (!spec) return p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed");
   const void* plane = src;
   if (!src_on_device) {
@@ -639,6 +776,7 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
       (void)hipGetLastError();
       p->ring.erase(fn);
       p->retire(spec);
+      p->end_call(s);
       return p->fail(FFT3D_ELAUNCH, "plane upload failed");
     }
     plane = p->dStage;
@@ -648,11 +786,22 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
   hipEvent_t t0 = p->prof_begin(s);
   FK_LAUNCH(k, dim3(grid), dim3(64), s, p->args, plane, spec);
   p->prof_end(t0, fft3d_plane::K_ANALYSIS, s);
+  p->end_call(s);
   if (!p->launched("fft2d forward")) {
     p->ring.erase(fn);
     p->retire(spec);
     return FFT3D_ELAUNCH;
   }
+  return FFT3D_OK;
+}
+
+int fft3d_plane_prefetch(fft3d_plane* p, int fn, const void* src, size_t bytes) {
+  if (!p || !src) return FFT3D_EINVAL;
+  const size_t need = (size_t)p->d.width * p->d.height * p->bps;
+  if (bytes < need) return p->fail(FFT3D_EINVAL, "plane buffer too small");
+  // carried out by the next process / forward_process call, after its launches
+  p->pendFn = p->ring.count(fn) ? -1 : fn;
+  p->pendSrc = src;
   return FFT3D_OK;
 }
 
@@ -662,7 +811,15 @@ int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst, 
   fk::WindowArgs w{};
   int rc = p->window(window, nwin, bytes, w);
   if (rc != FFT3D_OK) return rc;
-  return p->run(w, dst, dst_on_device, p->stream(stream), nullptr, nullptr, 0);
+  hipStream_t s = p->stream(stream);
+  if (!p->wait_prefetched(window, nwin, s)) {
+    (void)hipGetLastError();
+    return p->fail(FFT3D_ELAUNCH, "wait for prefetched spectra failed");
+  }
+  rc = p->run(w, dst, dst_on_device, s, nullptr, nullptr, 0);
+  p->pendFn = -1;  // a request the run did not reach is dropped with its buffer
+  p->end_call(s);
+  return rc;
 }
 
 int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t src_bytes,
@@ -692,15 +849,25 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
       return p->fail(FFT3D_ENOMEM, "GPU alloc failed (upload staging)");
     if (p->upload(p->dStageIn, src, need, s) != hipSuccess) {
       (void)hipGetLastError();
+      p->end_call(s);
       return p->fail(FFT3D_ELAUNCH, "plane upload failed");
     }
     plane = p->dStageIn;
   }
-  fk::c32* spec = p->spectrum(fn);
-  if (!spec) return p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed");
+  fk::c32* spec = p->spectrum(fn, s);
+  if (!spec) {
+    p->end_call(s);
+    return p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed");
+  }
   fk::WindowArgs w{};
   int rc = p->window(window, nwin, dst_bytes, w);
+  if (rc == FFT3D_OK && !p->wait_prefetched(window, nwin, s)) {
+    (void)hipGetLastError();
+    rc = p->fail(FFT3D_ELAUNCH, "wait for prefetched spectra failed");
+  }
   if (rc == FFT3D_OK) rc = p->run(w, dst, dst_on_device, s, plane, spec, mask);
+  p->pendFn = -1;
+  p->end_call(s);
   if (rc != FFT3D_OK) {
     // the spectrum may be incomplete: drop it, as a failed forward does
     p->ring.erase(fn);
@@ -711,11 +878,14 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
 
 void fft3d_plane_prune(fft3d_plane* p, int lo, int hi) {
   if (!p) return;
+  if (p->pendFn >= 0 && !((lo < 0 || p->pendFn >= lo) && (hi < 0 || p->pendFn <= hi)))
+    p->pendFn = -1;
   for (auto it = p->ring.begin(); it != p->ring.end();) {
     const bool keep = (lo < 0 || it->first >= lo) && (hi < 0 || it->first <= hi);
     if (keep) {
       ++it;
     } else {
+      p->forget(it->first);
       p->retire(it->second);
       it = p->ring.erase(it);
     }
@@ -726,6 +896,8 @@ void fft3d_plane_clear(fft3d_plane* p) {
   if (!p) return;
   for (auto& kv : p->ring) p->retire(kv.second);
   p->ring.clear();
+  p->forget_all();
+  p->pendFn = -1;
 }
 
 int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
@@ -754,6 +926,8 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
     for (auto* m : p->pool) p->release(m);
     p->ring.clear();
     p->pool.clear();
+    p->forget_all();
+    p->pendFn = -1;
     p->half = value == 1;
     p->specBytes = (p->half ? 4 : 8) * (size_t)p->nb * p->p.bh * (p->p.bw / 2);
     return FFT3D_OK;
@@ -788,6 +962,7 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
       return p->fail(FFT3D_ELAUNCH, "synchronize failed");
     }
     p->stg.release();
+    p->stgUp.release();
     p->stg.chunk = chunk;
     return FFT3D_OK;
   }

@@ -4,12 +4,16 @@
 // the plane goes through a few pinned chunks owned by the plane object. Host
 // threads copy chunk i between the caller's buffer and a pinned chunk while
 // the DMA engine moves its neighbours:
-//   upload   : copy chunk i into pinned slot i % NBUF, queue its DMA, go on
+//   upload   : copy chunk i into pinned slot i % nbuf, queue its DMA, go on
 //              (a slot is refilled once the DMA that read it has finished);
-//   download : queue the DMAs of the first NBUF chunks, then for each chunk
+//   download : queue the DMAs of the first nbuf chunks, then for each chunk
 //              wait for its DMA, copy it out, and queue the DMA of the chunk
-//              NBUF ahead into the slot just emptied.
+//              nbuf ahead into the slot just emptied.
 // Pinned (page-locked) and device buffers go to the DMA engine as they are.
+// Device -> pinned host copies run as a kernel (k_copy_out) on the caller's
+// stream rather than on the DMA engine: on this platform a host-to-device DMA
+// and a device-to-host DMA do not overlap, but a copy kernel writing host
+// memory overlaps a host-to-device DMA (fft3d_plane_prefetch relies on it).
 //
 // The reference moves each plane with one blocking clEnqueueWriteBuffer /
 // clEnqueueReadBuffer (src/cl/gpu.cpp:360-365, src/fft3dcl.cpp:396-448).
@@ -24,6 +28,36 @@
 #include <vector>
 
 namespace fx {
+
+// dst[i] = src[i] for i < n16 (16-byte units): device memory to page-locked
+// host memory, written by vector stores over the bus.
+__global__ __launch_bounds__(256) void k_copy_out(const uint4* __restrict__ src,
+                                                  uint4* __restrict__ dst, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+// Device -> page-locked host on stream s (asynchronous). The kernel writes
+// through the device address the runtime reports for h; host memory without
+// one, and sizes or addresses that are not multiples of 16, go to
+// hipMemcpyAsync.
+inline hipError_t copy_to_host(void* h, const void* d, size_t n, hipStream_t s) {
+  hipPointerAttribute_t a{};
+  void* dh = nullptr;
+  if (hipPointerGetAttributes(&a, h) == hipSuccess && a.type == hipMemoryTypeHost)
+    dh = a.devicePointer;
+  else
+    (void)hipGetLastError();
+  if (!dh || n % 16 || reinterpret_cast<uintptr_t>(dh) % 16 ||
+      reinterpret_cast<uintptr_t>(d) % 16)
+    return hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, s);
+  const size_t n16 = n / 16;
+  const unsigned grid = (unsigned)std::min<size_t>(1024, (n16 + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  FK_LAUNCH(k_copy_out, dim3(grid), dim3(256), s, static_cast<const uint4*>(d),
+            static_cast<uint4*>(dh), n16);
+  return hipGetLastError();
+}
 
 // Process-wide pool of host threads for large memcpy: the caller and the
 // helpers each copy one contiguous piece. One copy at a time; a caller that
@@ -114,18 +148,23 @@ inline bool pageable(const void* p) {
   return a.type == hipMemoryTypeUnregistered;
 }
 
-// Pinned chunks of one plane and the events of the DMAs that use them.
+// Pinned chunks of one plane and the events of the DMAs that use them. With
+// nbuf chunks covering a whole plane, an upload returns as soon as the host
+// copies are done; with fewer, it also waits for the DMAs of earlier chunks.
 struct Staging {
-  static constexpr int NBUF = 3;
+  static constexpr int MAXBUF = 16;
   static constexpr size_t kDefaultChunk = 4 << 20;
   size_t chunk = kDefaultChunk;
-  void* pin[NBUF] = {};
-  hipEvent_t ev[NBUF] = {};
-  bool inFlight[NBUF] = {};
+  int nbuf = 3;
+  void* pin[MAXBUF] = {};
+  hipEvent_t ev[MAXBUF] = {};
+  bool inFlight[MAXBUF] = {};
 
   bool ready() const { return pin[0] != nullptr; }
-  hipError_t init() {
-    for (int k = 0; k < NBUF; k++) {
+  // n chunks (1..MAXBUF), allocated now
+  hipError_t init(int n = 3) {
+    nbuf = std::max(1, std::min(n, MAXBUF));
+    for (int k = 0; k < nbuf; k++) {
       hipError_t e = hipHostMalloc(&pin[k], chunk, hipHostMallocDefault);
       if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
       if (e != hipSuccess) {
@@ -137,7 +176,7 @@ struct Staging {
   }
   // the caller makes sure no DMA on these chunks is still queued
   void release() {
-    for (int k = 0; k < NBUF; k++) {
+    for (int k = 0; k < MAXBUF; k++) {
       if (pin[k]) (void)hipHostFree(pin[k]);
       if (ev[k]) (void)hipEventDestroy(ev[k]);
       pin[k] = nullptr;
@@ -152,7 +191,8 @@ struct Staging {
   }
   hipError_t queue(int k, void* dst, const void* src, size_t n, hipMemcpyKind kind,
                    hipStream_t s) {
-    hipError_t e = hipMemcpyAsync(dst, src, n, kind, s);
+    hipError_t e = kind == hipMemcpyDeviceToHost ? copy_to_host(dst, src, n, s)
+                                                 : hipMemcpyAsync(dst, src, n, kind, s);
     if (e == hipSuccess) e = hipEventRecord(ev[k], s);
     if (e == hipSuccess) inFlight[k] = true;
     return e;
@@ -164,7 +204,7 @@ struct Staging {
     char* d = static_cast<char*>(dDst);
     const char* h = static_cast<const char*>(hSrc);
     for (size_t o = 0, i = 0; o < n; o += chunk, i++) {
-      const int k = (int)(i % NBUF);
+      const int k = (int)(i % nbuf);
       const size_t len = std::min(chunk, n - o);
       hipError_t e = wait(k);
       if (e != hipSuccess) return e;
@@ -182,23 +222,23 @@ struct Staging {
     const char* d = static_cast<const char*>(dSrc);
     const size_t nch = (n + chunk - 1) / chunk;
     auto issue = [&](size_t i) {
-      const int k = (int)(i % NBUF);
+      const int k = (int)(i % nbuf);
       hipError_t e = wait(k);  // an upload's DMA may still read the slot
       if (e != hipSuccess) return e;
       const size_t o = i * chunk;
       return queue(k, pin[k], d + o, std::min(chunk, n - o), hipMemcpyDeviceToHost, s);
     };
-    for (size_t i = 0; i < nch && i < (size_t)NBUF; i++) {
+    for (size_t i = 0; i < nch && i < (size_t)nbuf; i++) {
       const hipError_t e = issue(i);
       if (e != hipSuccess) return e;
     }
     for (size_t i = 0; i < nch; i++) {
-      const int k = (int)(i % NBUF);
+      const int k = (int)(i % nbuf);
       hipError_t e = wait(k);
       if (e != hipSuccess) return e;
       const size_t o = i * chunk;
       copy_pool().copy(h + o, pin[k], std::min(chunk, n - o));
-      if (i + NBUF < nch && (e = issue(i + NBUF)) != hipSuccess) return e;
+      if (i + nbuf < nch && (e = issue(i + nbuf)) != hipSuccess) return e;
     }
     return hipSuccess;
   }

@@ -16,6 +16,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <iterator>
+#include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -210,15 +213,26 @@ inline hipError_t hipEventCreate(hipEvent_t* e) {
 }
 #define hipEventDisableTiming 2
 inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
-// host memory: every buffer of the emulation is plain host memory; the
-// pointer query reports all of them as pageable (the staged transfer path)
+// host memory: every buffer of the emulation is plain host memory. The
+// pointer query reports the blocks of hipHostMalloc as page-locked (device
+// address = host address) and everything else as pageable, so the staged
+// transfer path and its copy kernel both run.
 #define hipHostMallocDefault 0
+inline std::mutex emu_pinned_mtx_;
+inline std::map<uintptr_t, size_t> emu_pinned_;  // base -> size
 inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) {
   if (emu_fail_alloc_) return hipErrorOutOfMemory;
   *p = std::calloc(1, n ? n : 1);
-  return *p ? hipSuccess : hipErrorOutOfMemory;
+  if (!*p) return hipErrorOutOfMemory;
+  std::lock_guard<std::mutex> lk(emu_pinned_mtx_);
+  emu_pinned_[reinterpret_cast<uintptr_t>(*p)] = n ? n : 1;
+  return hipSuccess;
 }
 inline hipError_t hipHostFree(void* p) {
+  {
+    std::lock_guard<std::mutex> lk(emu_pinned_mtx_);
+    emu_pinned_.erase(reinterpret_cast<uintptr_t>(p));
+  }
   std::free(p);
   return hipSuccess;
 }
@@ -226,15 +240,25 @@ enum hipMemoryType { hipMemoryTypeUnregistered = 0, hipMemoryTypeHost = 1, hipMe
 struct hipPointerAttribute_t {
   hipMemoryType type;
   int device;
+  void* devicePointer;
 };
-inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t* a, const void*) {
+inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t* a, const void* p) {
   a->type = hipMemoryTypeUnregistered;
   a->device = 0;
+  a->devicePointer = nullptr;
+  const uintptr_t u = reinterpret_cast<uintptr_t>(p);
+  std::lock_guard<std::mutex> lk(emu_pinned_mtx_);
+  auto it = emu_pinned_.upper_bound(u);
+  if (it != emu_pinned_.begin() && u < std::prev(it)->first + std::prev(it)->second) {
+    a->type = hipMemoryTypeHost;
+    a->devicePointer = const_cast<void*>(p);
+  }
   return hipSuccess;
 }
 inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
   *ms = 0.0f;
   return hipSuccess;

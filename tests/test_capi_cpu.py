@@ -267,7 +267,7 @@ def test_emu_forward_process_equals_forward_then_process(emu_lib, kind, kw, W, H
         assert np.array_equal(res[0][1][fn], res[1][1][fn])
 
 
-@pytest.mark.parametrize("chunk,fuse", [(256, False), (4160, True), (0, False)])
+@pytest.mark.parametrize("chunk,fuse", [(2112, False), (4160, True), (0, False)])
 def test_emu_staged_host_transfers(emu_lib, chunk, fuse):
     """Pageable host buffers through the pinned chunks (many chunks with a
     ragged last one, or one chunk split over the copy threads: the plane is
@@ -288,6 +288,74 @@ def test_emu_staged_host_transfers(emu_lib, chunk, fuse):
         outs.append(out)
     assert outs[0].any()
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("kw,order,fuse", [
+    (dict(bt=4, sharpen=0.4), list(range(6)), False),
+    (dict(bt=3), [0, 1, 2, 4, 3, 5], True),                # out of order: prefetch wasted
+    (dict(bt=2, bw=16, bh=16), list(range(6)), False),
+    (dict(bt=1), list(range(4)), False),                   # one-frame window: no prefetch
+])
+def test_emu_prefetch_equals_plain(emu_lib, kw, order, fuse):
+    """Requests with the next frame prefetched give the same frames and ring
+    spectra as plain requests."""
+    from fft3dcl_amd.filter import Instance
+    W, H = 200, 120
+    planes = make_clip(17, W, H, 6, "u10")
+    res = []
+    for prefetch in (True, False):
+        inst = Instance(1, W, H, 10, 0, 6, lib=emu_lib, fuse=fuse, prefetch=prefetch, **kw)
+        out = np.zeros_like(planes[0])
+        specs = {}
+        for n in order:
+            inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
+            specs.update({fn: inst.objs[0].read_spectrum(fn) for fn in range(6)
+                          if inst.objs[0].has(fn) and fn not in specs})
+        inst.close()
+        res.append((out, specs))
+    assert res[0][0].any()
+    assert np.array_equal(res[0][0], res[1][0])
+    for fn in res[0][1].keys() & res[1][1].keys():
+        assert np.array_equal(res[0][1][fn], res[1][1][fn])
+
+
+def test_emu_prefetch_calls(emu_lib):
+    """A prefetch request is carried out by the next process call, lands the
+    same spectrum as forward, and is dropped by a prune that excludes it."""
+    planes = make_clip(18, 96, 64, 6, "u8")
+    params = capi.make_params(bt=3)
+    pl = capi.Plane(params, 96, 64, 8, lib=emu_lib)
+    ref = capi.Plane(params, 96, 64, 8, lib=emu_lib)
+    out = np.zeros((64, 96), np.uint8)
+    for k in range(3):
+        pl.forward(k, planes[0][k])
+    pl.prefetch(3, planes[0][3])
+    assert not pl.has(3)
+    pl.process([0, 1, 2], out)
+    assert pl.has(3)
+    ref.forward(3, planes[0][3])
+    want = ref.read_spectrum(3)
+    assert np.array_equal(pl.read_spectrum(3), want)
+    pl.prefetch(3, planes[0][1])  # already in the ring: no-op
+    pl.process([1, 2, 3], out)
+    assert np.array_equal(pl.read_spectrum(3), want)
+    pl.prefetch(4, planes[0][4])
+    pl.prune(1, 3)                # excludes 4: the request goes
+    pl.process([1, 2, 3], out)
+    assert not pl.has(4)
+    pl.prefetch(5, planes[0][5])
+    pl.forward(5, planes[0][5])   # forwarded in between: the request finds it there
+    pl.process([2, 3, 5], out)
+    ref.forward(5, planes[0][5])
+    assert np.array_equal(pl.read_spectrum(5), ref.read_spectrum(5))
+    rc = pl.lib.fft3d_plane_prefetch(pl.h, 1, planes[0][1].ctypes.data, 10)
+    assert rc == -1 and "too small" in pl.error
+    with pytest.raises(ValueError):
+        class Dev:
+            is_cuda = True
+        pl.prefetch(1, Dev())
+    pl.close()
+    ref.close()
 
 
 def test_emu_staging_options(emu_lib):

@@ -3,7 +3,8 @@ emulation of the HIP kernels) under AddressSanitizer in a child process, so an
 out-of-bounds global or LDS access in the kernels' index arithmetic fails on
 the CPU before it could fault a GPU. The host buffers go through the pinned
 chunks of the staged transfers (ragged chunks, and the threaded copy of a
-plane above 256 KiB). LDS arrays are function statics in the
+plane above 256 KiB), and with the sweep on each request prefetches the next
+frame. LDS arrays are function statics in the
 emulation, so their overruns are caught as global-buffer overflows."""
 import os
 import subprocess
@@ -30,9 +31,9 @@ for kind, bits, st, kw in cases:
     for W, H in ((83, 45), (300, 70)):
         planes = make_clip(1, W, H, 3, kind)
         for sweep in (1, 0):
-            inst = Instance(1, W, H, bits, st, 3, lib=lib, **kw)
+            inst = Instance(1, W, H, bits, st, 3, lib=lib, prefetch=bool(sweep), **kw)
             inst.objs[0].set_option(1, sweep)
-            inst.objs[0].set_option(7, 320 if sweep else 0)  # pinned chunks of 320 bytes
+            inst.objs[0].set_option(7, 8192 if sweep else 0)  # pinned chunks of 8 KiB
             out = np.zeros_like(planes[0])
             for n in range(3):
                 inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])

@@ -135,6 +135,64 @@ def test_device_resident_path_equals_host_path(gpu):
     assert np.array_equal(dev_out.cpu().numpy().view(np.uint16), host)
 
 
+@pytest.mark.parametrize("pinned", [False, True])
+def test_prefetch_equals_plain(gpu, pinned):
+    """Sequential requests with the next new frame prefetched on the upload
+    stream give the same frames as plain requests (4K, host buffers)."""
+    W, H, n = 3840, 2160, 8
+    frames = make_plane_sequence(12, W, H, n, "u10")
+    src = torch.from_numpy(frames).pin_memory() if pinned else frames
+    outs = []
+    for prefetch in (True, False):
+        inst = Instance(1, W, H, 10, 0, n, bt=4, sharpen=0.4, lib=gpu, prefetch=prefetch)
+        if pinned:
+            out = torch.zeros((n, H, W), dtype=torch.int16).pin_memory()
+        else:
+            out = np.zeros_like(frames)
+        for k in range(n):
+            inst.get_frame(k, [lambda fn: src[fn]], [out[k]])
+        inst.close()
+        outs.append(out.numpy().view(np.uint16) if pinned else out)
+    assert outs[1].any()
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("dst_on_device", [False, True])
+def test_prefetched_frame_used_at_once(gpu, dst_on_device):
+    """A process call whose window holds a frame prefetched by the call just
+    before waits for that prefetch on its own stream (host or device output;
+    with device output nothing blocks in between)."""
+    W, H = 3840, 2160
+    frames = make_plane_sequence(13, W, H, 6, "u10")
+    params = capi.make_params(bt=4)
+    res = []
+    for pre in (True, False):
+        pl = capi.Plane(params, W, H, 10, lib=gpu)
+        # a stream of its own (a null handle would select the plane's stream)
+        ts = torch.cuda.Stream() if dst_on_device else None
+        stream = ts.cuda_stream if dst_on_device else None
+        outs = [torch.zeros((H, W), dtype=torch.int16, device="cuda") if dst_on_device
+                else np.zeros((H, W), np.uint16) for _ in range(3)]
+        torch.cuda.synchronize()
+        for k in range(4):
+            pl.forward(k, frames[k], False, stream)
+        for i, k in enumerate((4, 5)):
+            if pre:
+                pl.prefetch(k, frames[k])   # carried out by this process call
+            else:
+                pl.forward(k, frames[k], False, stream)
+            pl.process([k - 4, k - 3, k - 2, k - 1], outs[i], dst_on_device, stream)
+        pl.process([2, 3, 4, 5], outs[2], dst_on_device, stream)
+        if dst_on_device:
+            ts.synchronize()
+            outs = [o.cpu().numpy().view(np.uint16) for o in outs]
+        pl.close()
+        res.append(outs)
+    for a, b in zip(*res):
+        assert a.any()
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("chunk,fuse", [(0, False), (1 << 20, False), (4160, True)])
 def test_host_transfers_staged_direct_pinned(gpu, chunk, fuse):
     """The same frames whichever way host buffers travel: pageable numpy
