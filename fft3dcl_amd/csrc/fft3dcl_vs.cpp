@@ -41,6 +41,8 @@ struct Inst {
   std::vector<fft3d_plane*> gpu;        // parallel to planes
   std::vector<uint8_t> staging;         // input plane whose rows are padded
   std::vector<uint8_t> stagingOut;      // output plane whose rows are padded
+  std::vector<uint8_t> stagingNext;     // prefetched plane whose rows are padded
+  bool prefetch = false;                // FFT3DCL_PREFETCH=1 (bt >= 2 only)
   std::mutex mtx;
   ~Inst() {
     for (auto* p : gpu) fft3d_plane_destroy(p);
@@ -71,6 +73,14 @@ void logUsage(VSCore* core, const VSAPI* vsapi, const char* what) {
 
 int clampFrame(int n, int numFrames) { return n < 0 ? 0 : (n > numFrames - 1 ? numFrames - 1 : n); }
 
+// With FFT3DCL_PREFETCH=1, the frame a sequential request for n+1 adds to the
+// window is requested along with n's window and prefetched (-1: none).
+int nextNewFrame(const Inst* d, int n) {
+  if (!d->prefetch || d->bm < 2) return -1;
+  const int f = n - d->current + d->bt;
+  return f >= 0 && f < d->vi.numFrames ? f : -1;
+}
+
 const VSFrame* getFrame(int n, int reason, void* instanceData, void** /*frameData*/,
                         VSFrameContext* ctx, VSCore* core, const VSAPI* vsapi) {
   Inst* d = static_cast<Inst*>(instanceData);
@@ -80,6 +90,8 @@ const VSFrame* getFrame(int n, int reason, void* instanceData, void** /*frameDat
       const int lo = clampFrame(n - d->current, vi.numFrames);
       const int hi = clampFrame(n - d->current + d->bt - 1, vi.numFrames);
       for (int f = lo; f <= hi; f++) vsapi->requestFrameFilter(f, d->clip, ctx);
+      const int nx = nextNewFrame(d, n);
+      if (nx > hi) vsapi->requestFrameFilter(nx, d->clip, ctx);
     } else {
       vsapi->requestFrameFilter(clampFrame(n, vi.numFrames), d->clip, ctx);
     }
@@ -103,6 +115,8 @@ const VSFrame* getFrame(int n, int reason, void* instanceData, void** /*frameDat
   }
 
   std::lock_guard<std::mutex> lock(d->mtx);
+  const int nx = nextNewFrame(d, n);
+  const VSFrame* frNext = nullptr;  // held until the process calls have returned
   for (size_t k = 0; k < d->planes.size(); k++) {
     const int p = d->planes[k];
     fft3d_plane* g = d->gpu[k];
@@ -129,6 +143,20 @@ const VSFrame* getFrame(int n, int reason, void* instanceData, void** /*frameDat
         std::memcpy(d->staging.data() + y * rowBytes, sp + y * ss, rowBytes);
       return d->staging.data();
     };
+    // the next sequential request's new frame: the process call below uploads
+    // and transforms it alongside its own work
+    if (nx >= 0 && !fft3d_plane_has(g, nx)) {
+      if (!frNext) frNext = vsapi->getFrameFilter(nx, d->clip, ctx);
+      const uint8_t* sp = vsapi->getReadPtr(frNext, p);
+      const ptrdiff_t ss = vsapi->getStride(frNext, p);
+      if ((size_t)ss != rowBytes) {
+        d->stagingNext.resize(planeBytes);
+        for (int y = 0; y < ph; y++)
+          std::memcpy(d->stagingNext.data() + y * rowBytes, sp + y * ss, rowBytes);
+        sp = d->stagingNext.data();
+      }
+      (void)fft3d_plane_prefetch(g, nx, sp, planeBytes);  // a failure only loses the overlap
+    }
     // the missing frames but the last go through forward; the last one is
     // transformed by the filter pass itself (forward_process)
     for (int k = 0; k + 1 < nmiss && rc == FFT3D_OK; k++) {
@@ -159,6 +187,7 @@ const VSFrame* getFrame(int n, int reason, void* instanceData, void** /*frameDat
                         fft3d_plane_error(g) +
                         " (this build has no CPU backend to fall back to)";
       vsapi->setFilterError(msg.c_str(), ctx);
+      if (frNext) vsapi->freeFrame(frNext);
       vsapi->freeFrame(dst);
       vsapi->freeFrame(src);
       return nullptr;
@@ -169,6 +198,7 @@ const VSFrame* getFrame(int n, int reason, void* instanceData, void** /*frameDat
     else if (d->bm != 0)
       fft3d_plane_clear(g);
   }
+  if (frNext) vsapi->freeFrame(frNext);
   vsapi->freeFrame(src);
   return dst;
 }
@@ -307,6 +337,9 @@ void create(const VSMap* in, VSMap* out, void* /*userData*/, VSCore* core, const
   d->bm = info.bm;
   d->bt = info.bt;
   d->current = info.current;
+  // opt-in: request and prefetch the frame a sequential next request adds
+  const char* pf = std::getenv("FFT3DCL_PREFETCH");
+  d->prefetch = pf && std::strcmp(pf, "1") == 0;
 
   VSFilterDependency deps[] = {{clip, rpGeneral}};
   // Kalman (bt=0) carries state from frame to frame: frames in order, one at a time.

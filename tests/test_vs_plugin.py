@@ -88,6 +88,26 @@ def test_request_window_and_filter_mode(host, bt, current):
         assert reqs[k] == want
 
 
+@pytest.mark.parametrize("bt,current,pad", [(3, 1, 0), (4, 2, 16)])
+def test_prefetch_env(host, monkeypatch, bt, current, pad):
+    """FFT3DCL_PREFETCH=1: each request also asks for the frame the next
+    sequential request adds, the plugin prefetches it (through its own copy
+    when rows are padded), and the output is unchanged."""
+    n = 7
+    planes = make_clip(5, 96, 64, n, "u8")
+    rc0, outs0, _, _, err0, _ = host.run(str(EMU_PLUGIN), planes, bt=bt, pad=pad)
+    monkeypatch.setenv("FFT3DCL_PREFETCH", "1")
+    rc1, outs1, reqs, _, err1, _ = host.run(str(EMU_PLUGIN), planes, bt=bt, pad=pad)
+    assert rc0 == 0 and rc1 == 0, err0 + err1
+    for f in range(n):
+        want = {min(max(f - current + i, 0), n - 1) for i in range(bt)}
+        if f - current + bt < n:
+            want.add(f - current + bt)
+        assert reqs[f] == sorted(want)
+    assert outs0[0].any()
+    assert np.array_equal(outs0[0], outs1[0])
+
+
 CASES = [("u8", dict(bt=3), 1, 0), ("u10", dict(bt=4, sharpen=0.4, sigma=3.0), 1, 16),
          ("u8", dict(bt=0), 1, 0), ("f32", dict(bt=2, sigma2=3.0), 1, 8),
          ("u8", dict(plane=4, bt=3), 3, 0), ("u10", dict(plane=1, bt=4), 3, 24),
@@ -123,3 +143,12 @@ def test_plugin_output_matches_oracle_emu(host, oracle, kind, kw, cf, pad):
 @pytest.mark.parametrize("kind,kw,cf,pad", CASES)
 def test_plugin_output_matches_oracle_gpu(gpu, host, oracle, kind, kw, cf, pad):
     run_vs_vs_oracle(host, oracle, HIP_PLUGIN, kind, kw, cf, pad, W=640, H=360, n=6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,kw,cf,pad", [CASES[1], CASES[5]])
+def test_plugin_prefetch_matches_oracle_gpu(gpu, host, oracle, monkeypatch, kind, kw, cf, pad):
+    """The shipped plugin with FFT3DCL_PREFETCH=1 on the GPU (upload stream,
+    events, copy-kernel readback) against the oracle."""
+    monkeypatch.setenv("FFT3DCL_PREFETCH", "1")
+    run_vs_vs_oracle(host, oracle, HIP_PLUGIN, kind, kw, cf, pad, W=640, H=360, n=8)
