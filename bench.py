@@ -206,6 +206,8 @@ def main():
                     help="skip the 32-stream served-form sample")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP16-spectra-ring variant of the workload")
+    ap.add_argument("--host-only", action="store_true",
+                    help="only measure the host-buffer rates (one JSON line of them)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, --steps times (the "
                          "reference point for HBM byte counters)")
@@ -236,6 +238,9 @@ def main():
 
     capi.dev_init(0)
     frames = make_plane_sequence(SEED, W, H, NUNIQ, "u10")
+    if args.host_only:
+        print(json.dumps({"host_buffers_pcie_inclusive": host_buffer_rates(frames)}), flush=True)
+        return
     dev = torch.from_numpy(frames).to("cuda")
     out = torch.empty((H, W), dtype=torch.int16, device="cuda")
     nframes = max(120, args.warmup + args.steps + 8)
