@@ -164,6 +164,46 @@ def fp16_ring_variant(dev, kb_fn, steps=100, warmup=10):
     return res
 
 
+def prefetch_variant(dev, steps=100, warmup=10):
+    """The same device-resident workload with each request prefetching the
+    next one's new frame (fft3d_plane_prefetch with a device source): that
+    frame's analysis runs on the plane's upload stream beside the current
+    sweep. Frames/s of both forms, and the largest difference from plain
+    requests over 8 frames, in LSB (0: the same kernels on the same data).
+    At 4K it is slower: the concurrent analysis takes the CU slots that the
+    sweep's two workgroups per CU need."""
+    import numpy as np
+    import torch
+    from fft3dcl_amd.filter import GRAY, Instance
+    outs, res = {}, {}
+    for pf in (False, True):
+        inst = Instance(GRAY, W, H, BITS, 0, steps + warmup + 16, prefetch=pf, **PARAMS)
+        stream = torch.cuda.Stream()
+        out = torch.empty((H, W), dtype=torch.int16, device="cuda")
+        srcs = [lambda fn: dev[fn % NUNIQ]]
+        frames = []
+        for n in range(8):
+            inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
+            with torch.cuda.stream(stream):
+                frames.append(out.clone())
+        for n in range(8, 8 + warmup):
+            inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for n in range(8 + warmup, 8 + warmup + steps):
+            inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        inst.close()
+        outs[pf] = torch.stack(frames).cpu().numpy().view(np.uint16).astype(np.int32)
+        ms = e0.elapsed_time(e1) / steps
+        res["prefetch" if pf else "plain"] = {"frames_per_s": round(1000.0 / ms, 1),
+                                              "ms_per_step": round(ms, 4)}
+    res["max_lsb_vs_plain"] = int(np.abs(outs[True] - outs[False]).max())
+    return res
+
+
 def served_sample(seconds=0.5, nstreams=32, uniq=3):
     """The served form (32 seeded streams of mixed resolution, bit depth, bt
     and block size, Poisson arrivals at 24-60 frames/s each over `seconds`):
@@ -309,6 +349,7 @@ def main():
     }
     if not args.no_variants:
         line["config"]["fp16_spectra_ring"] = fp16_ring_variant(dev, kernel_bytes)
+        line["config"]["device_prefetch"] = prefetch_variant(dev)
     if not args.no_host:
         line["config"]["host_buffers_pcie_inclusive"] = host_buffer_rates(frames)
     if not args.no_served:

@@ -97,7 +97,7 @@ def load(path: str | Path | None = None) -> C.CDLL:
     L.fft3d_plane_process.argtypes = [vp, C.POINTER(C.c_int), i, vp, sz, i, vp]
     L.fft3d_plane_forward_process.argtypes = [vp, i, vp, sz, i, C.POINTER(C.c_int), i, vp, sz,
                                               i, vp]
-    L.fft3d_plane_prefetch.argtypes = [vp, i, vp, sz]
+    L.fft3d_plane_prefetch.argtypes = [vp, i, vp, sz, i]
     L.fft3d_plane_prune.argtypes = [vp, i, i]
     L.fft3d_plane_prune.restype = None
     L.fft3d_plane_clear.argtypes = [vp]
@@ -217,13 +217,15 @@ class Plane:
         finally:
             self._pending = None
 
-    def prefetch(self, fn: int, src):
-        """Requests host plane fn in the ring; the next process or
-        forward_process call carries it out alongside its own work (host
-        buffers only). src is kept alive until that call returns."""
-        if getattr(src, "is_cuda", False):
-            raise ValueError("prefetch takes host buffers")
-        self._check(self.lib.fft3d_plane_prefetch(self.h, fn, _ptr(src), self.nbytes))
+    def prefetch(self, fn: int, src, on_device: bool = False):
+        """Requests plane fn in the ring; the next process or forward_process
+        call carries it out alongside its own work. A host src is kept alive
+        until that call returns; a device src must stay unchanged until the
+        frame is used."""
+        if bool(getattr(src, "is_cuda", False)) != bool(on_device):
+            raise ValueError("prefetch: on_device does not match the buffer")
+        self._check(self.lib.fft3d_plane_prefetch(self.h, fn, _ptr(src), self.nbytes,
+                                                  int(on_device)))
         self._pending = src
 
     def prune(self, lo: int, hi: int):

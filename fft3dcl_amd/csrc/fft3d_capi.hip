@@ -138,6 +138,9 @@ struct fft3d_plane {
   std::vector<hipEvent_t> evSpare;
   int pendFn = -1;                    // requested prefetch, run by the next process call
   const void* pendSrc = nullptr;
+  bool pendOnDevice = false;
+  hipEvent_t callStart = nullptr;     // start of the call that runs the prefetch
+  bool callStarted = false;
   float* dBlocks = nullptr;  // windowed blocks awaiting overlap-add (unfused path)
   float4* dKalman = nullptr; // bm == 0 state
   bool kalmanFirst = true;
@@ -211,6 +214,7 @@ struct fft3d_plane {
     for (auto& kv : upReady) (void)hipEventDestroy(kv.second);
     for (auto e : evSpare) (void)hipEventDestroy(e);
     if (lastCall) (void)hipEventDestroy(lastCall);
+    if (callStart) (void)hipEventDestroy(callStart);
     if (upDone) (void)hipEventDestroy(upDone);
     if (up) (void)hipStreamDestroy(up);
     (void)hipFree(dBlocks);
@@ -411,6 +415,33 @@ struct fft3d_plane {
   int run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipStream_t s,
           const void* fwdPlane, fk::c32* fwdSpec, int fwdMask);
   void run_prefetch();
+  // the upload stream and its events (created on first use, after a device
+  // synchronisation so that everything queued before comes first)
+  bool ensure_up() {
+    if (up) return true;
+    bool ok = hipDeviceSynchronize() == hipSuccess &&
+              hipStreamCreateWithFlags(&up, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&lastCall, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&upDone, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&callStart, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventRecord(lastCall, up) == hipSuccess &&
+         hipEventRecord(upDone, up) == hipSuccess;
+    if (!ok) {
+      (void)hipGetLastError();
+      if (up) (void)hipStreamDestroy(up);
+      up = nullptr;
+    }
+    return ok;
+  }
+  // At the start of a process call on stream s with a prefetch pending: marks
+  // what the caller queued on s before the call (a device source is ready
+  // after it), without this call's own launches.
+  void begin_call(hipStream_t s) {
+    callStarted = false;
+    if (pendFn < 0 || !ensure_up()) return;
+    if (hipEventRecord(callStart, s) == hipSuccess) callStarted = true;
+    else (void)hipGetLastError();
+  }
 
   bool launched(const char* what) {
     hipError_t e = hipGetLastError();
@@ -473,37 +504,31 @@ int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipS
   return FFT3D_OK;
 }
 
-// The pending prefetch: frame pendFn from host plane pendSrc into the ring on
-// the upload stream. Runs inside a process call, after its launches, so that
-// the host copies overlap that call's kernels and the upload overlaps its
-// readback. A failure only drops the request (the frame is then forwarded
-// when a window first needs it).
+// The pending prefetch: frame pendFn from plane pendSrc into the ring on the
+// upload stream. Runs inside a process call, after its launches, so that the
+// host copies overlap that call's kernels and the upload (or, for a device
+// source, the analysis) overlaps its kernels and readback. A failure only
+// drops the request (the frame is then forwarded when a window first needs
+// it).
 void fft3d_plane::run_prefetch() {
   const int fn = pendFn;
   const void* src = pendSrc;
+  const bool onDevice = pendOnDevice;
+  const bool started = callStarted;
   pendFn = -1;
   pendSrc = nullptr;
-  if (fn < 0 || ring.count(fn)) return;
+  callStarted = false;
+  if (fn < 0 || ring.count(fn) || !ensure_up()) return;
   const size_t need = (size_t)d.width * d.height * bps;
-  if (!up) {
-    // first use: everything queued so far on any stream comes first
-    bool ok = hipDeviceSynchronize() == hipSuccess &&
-              hipStreamCreateWithFlags(&up, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&lastCall, hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&upDone, hipEventDisableTiming) == hipSuccess;
-    ok = ok && hipEventRecord(lastCall, up) == hipSuccess &&
-         hipEventRecord(upDone, up) == hipSuccess;
-    if (!ok) {
-      (void)hipGetLastError();
-      return;
-    }
-  }
-  // the plane's earlier calls may still read a pooled spectrum
-  if (hipStreamWaitEvent(up, lastCall, 0) != hipSuccess) {
+  // the plane's earlier calls may still read a pooled spectrum; a device
+  // source is complete once the caller's earlier work on its stream is
+  if (hipStreamWaitEvent(up, lastCall, 0) != hipSuccess ||
+      (started && hipStreamWaitEvent(up, callStart, 0) != hipSuccess)) {
     (void)hipGetLastError();
     return;
   }
-  if (!dStageUp && !alloc(&dStageUp, need)) return;
+  if (onDevice && !started) return;  // no event to order the source by
+  if (!onDevice && !dStageUp && !alloc(&dStageUp, need)) return;
   fk::c32* spec = spectrum(fn, up);
   if (!spec) {
     (void)hipGetLastError();
@@ -514,7 +539,7 @@ void fft3d_plane::run_prefetch() {
     ring.erase(fn);
     retire(spec);
   };
-  bool staged = stagingOn && fx::pageable(src);
+  bool staged = !onDevice && stagingOn && fx::pageable(src);
   if (staged && !stgUp.ready()) {
     // pinned chunks for the whole plane: the host copies never wait for a DMA
     stgUp.chunk = stg.chunk;
@@ -524,13 +549,16 @@ void fft3d_plane::run_prefetch() {
       staged = false;
     }
   }
-  const hipError_t e = staged ? stgUp.upload(dStageUp, src, need, up)
-                              : hipMemcpyAsync(dStageUp, src, need, hipMemcpyHostToDevice, up);
+  const hipError_t e =
+      onDevice ? hipSuccess
+               : (staged ? stgUp.upload(dStageUp, src, need, up)
+                         : hipMemcpyAsync(dStageUp, src, need, hipMemcpyHostToDevice, up));
   if (e != hipSuccess) return drop();
+  const void* plane = onDevice ? src : dStageUp;
   const int grid = (nb + ks.G - 1) / ks.G;
   AnalysisFn k = half ? ks.analysisH : ks.analysis;
   hipEvent_t t0 = prof_begin(up);
-  FK_LAUNCH(k, dim3(grid), dim3(64), up, args, dStageUp, spec);
+  FK_LAUNCH(k, dim3(grid), dim3(64), up, args, plane, spec);
   prof_end(t0, K_ANALYSIS, up);
   if (hipGetLastError() != hipSuccess) return drop();
   hipEvent_t ev = nullptr;
@@ -795,13 +823,15 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
   return FFT3D_OK;
 }
 
-int fft3d_plane_prefetch(fft3d_plane* p, int fn, const void* src, size_t bytes) {
+int fft3d_plane_prefetch(fft3d_plane* p, int fn, const void* src, size_t bytes,
+                         int src_on_device) {
   if (!p || !src) return FFT3D_EINVAL;
   const size_t need = (size_t)p->d.width * p->d.height * p->bps;
   if (bytes < need) return p->fail(FFT3D_EINVAL, "plane buffer too small");
   // carried out by the next process / forward_process call, after its launches
   p->pendFn = p->ring.count(fn) ? -1 : fn;
   p->pendSrc = src;
+  p->pendOnDevice = src_on_device != 0;
   return FFT3D_OK;
 }
 
@@ -810,9 +840,14 @@ int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst, 
   if (!p || !window || !dst) return FFT3D_EINVAL;
   fk::WindowArgs w{};
   int rc = p->window(window, nwin, bytes, w);
-  if (rc != FFT3D_OK) return rc;
+  if (rc != FFT3D_OK) {
+    p->pendFn = -1;
+    return rc;
+  }
   hipStream_t s = p->stream(stream);
+  p->begin_call(s);
   if (!p->wait_prefetched(window, nwin, s)) {
+    p->pendFn = -1;
     (void)hipGetLastError();
     return p->fail(FFT3D_ELAUNCH, "wait for prefetched spectra failed");
   }
@@ -829,35 +864,41 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
   int mask = 0;
   for (int i = 0; i < nwin && i < 4; i++)
     if (window[i] == fn) mask |= 1 << i;
+  // a pending prefetch that this call does not reach is dropped
+  auto bail = [p](int rc) {
+    p->pendFn = -1;
+    return rc;
+  };
   if (!mask || !(p->sweepOk && p->sweepOn && p->fuseOn) || p->half) {
     const int rc = fft3d_plane_forward(p, fn, src, src_bytes, src_on_device, stream);
-    if (rc != FFT3D_OK) return rc;
+    if (rc != FFT3D_OK) return bail(rc);
     return fft3d_plane_process(p, window, nwin, dst, dst_bytes, dst_on_device, stream);
   }
   const size_t need = (size_t)p->d.width * p->d.height * p->bps;
-  if (src_bytes < need) return p->fail(FFT3D_EINVAL, "plane buffer too small");
+  if (src_bytes < need) return bail(p->fail(FFT3D_EINVAL, "plane buffer too small"));
   if (nwin != (p->p.bm >= 2 ? p->p.bt : 1))
-    return p->fail(FFT3D_EINVAL, "window length does not match bt");
-  if (dst_bytes < need) return p->fail(FFT3D_EINVAL, "output buffer too small");
+    return bail(p->fail(FFT3D_EINVAL, "window length does not match bt"));
+  if (dst_bytes < need) return bail(p->fail(FFT3D_EINVAL, "output buffer too small"));
   for (int i = 0; i < nwin; i++)
     if (window[i] != fn && !p->ring.count(window[i]))
-      return p->fail(FFT3D_EMISSING, "missing spectra in window");
+      return bail(p->fail(FFT3D_EMISSING, "missing spectra in window"));
   hipStream_t s = p->stream(stream);
+  p->begin_call(s);
   const void* plane = src;
   if (!src_on_device) {
     if (!p->dStageIn && !p->alloc(&p->dStageIn, need))
-      return p->fail(FFT3D_ENOMEM, "GPU alloc failed (upload staging)");
+      return bail(p->fail(FFT3D_ENOMEM, "GPU alloc failed (upload staging)"));
     if (p->upload(p->dStageIn, src, need, s) != hipSuccess) {
       (void)hipGetLastError();
       p->end_call(s);
-      return p->fail(FFT3D_ELAUNCH, "plane upload failed");
+      return bail(p->fail(FFT3D_ELAUNCH, "plane upload failed"));
     }
     plane = p->dStageIn;
   }
   fk::c32* spec = p->spectrum(fn, s);
   if (!spec) {
     p->end_call(s);
-    return p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed");
+    return bail(p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed"));
   }
   fk::WindowArgs w{};
   int rc = p->window(window, nwin, dst_bytes, w);

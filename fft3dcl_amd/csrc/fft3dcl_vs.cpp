@@ -155,7 +155,7 @@ const VSFrame* getFrame(int n, int reason, void* instanceData, void** /*frameDat
           std::memcpy(d->stagingNext.data() + y * rowBytes, sp + y * ss, rowBytes);
         sp = d->stagingNext.data();
       }
-      (void)fft3d_plane_prefetch(g, nx, sp, planeBytes);  // a failure only loses the overlap
+      (void)fft3d_plane_prefetch(g, nx, sp, planeBytes, 0);  // a failure only loses the overlap
     }
     // the missing frames but the last go through forward; the last one is
     // transformed by the filter pass itself (forward_process)

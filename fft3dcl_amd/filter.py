@@ -53,9 +53,9 @@ class Instance:
         """fuse: let the filter pass transform the last missing window frame
         itself (FFT3D_OPT_FUSED_ANALYSIS) instead of a separate analysis.
         half: store the spectra ring as FP16 (FFT3D_OPT_SPECTRA_FP16).
-        prefetch: with host buffers and bt >= 2, each request also transforms
-        the frame that the next sequential request adds to the window
-        (fft3d_plane_prefetch), so its upload overlaps this request's work."""
+        prefetch: with bt >= 2, each request also transforms the frame that
+        the next sequential request adds to the window (fft3d_plane_prefetch),
+        so its upload and analysis overlap this request's work."""
         device = int(kw.pop("device", 2))
         if device == 0:
             raise ValueError("FFT3DCL: device=0 selects the CPU backend, which this HIP "
@@ -98,11 +98,11 @@ class Instance:
         for k, q in enumerate(self.planes):
             obj = self.objs[k]
             win = self.window(n)
-            if self.prefetch and self.bm >= 2 and not on_device:
+            if self.prefetch and self.bm >= 2:
                 # the next sequential request's new frame (kept by the prune below)
                 nxt = clamp_frame(n - self.current + self.bt, self.num_frames)
                 if nxt not in win and not obj.has(nxt):
-                    obj.prefetch(nxt, src_planes[q](nxt))
+                    obj.prefetch(nxt, src_planes[q](nxt), on_device)
             missing = [fn for fn in dict.fromkeys(win) if not obj.has(fn)]
             for fn in missing[:-1]:
                 obj.forward(fn, src_planes[q](fn), on_device, stream)

@@ -126,22 +126,28 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
 
 /* forwardRaw(fn, bytes) issued ahead of need (src/cl/gpu.h, forwardRaw; the
  * host step src/fft3dcl.cpp:396-448 forwards each window frame in the call
- * that first needs it). Requests that host plane fn go into the ring; the
- * next fft3d_plane_process or fft3d_plane_forward_process call on the plane
- * carries it out after launching its own kernels and before its readback:
- * the host copy (pageable buffers go through pinned chunks that hold the
- * whole plane) overlaps that call's kernels, and the upload and analysis run
- * on the plane's own upload stream alongside its readback. src must stay
- * valid until that call returns. One request at a time (a new one replaces
- * it); a no-op when fn is already in the ring; dropped by a prune that
- * excludes fn, by fft3d_plane_clear, and by a call that fails before its
+ * that first needs it). Requests that plane fn go into the ring; the next
+ * fft3d_plane_process or fft3d_plane_forward_process call on the plane
+ * carries it out after launching its own kernels and before its readback,
+ * on the plane's own upload stream:
+ *   host src   : the host copy (pageable buffers go through pinned chunks
+ *                that hold the whole plane) overlaps that call's kernels;
+ *                the upload and the analysis run alongside its readback;
+ *   device src : the analysis runs alongside that call's kernels. It waits
+ *                for the work the caller queued on that call's stream before
+ *                the call, so src may be written there just before.
+ * src must stay valid, and a device src unchanged, until the prefetch is
+ * done: until the carrying call returns for a host src, until the next call
+ * whose window holds fn for a device src. One request at a time (a new one
+ * replaces it); a no-op when fn is already in the ring; dropped by a prune
+ * that excludes fn, by fft3d_plane_clear, and by a call that fails before its
  * readback. If the prefetch itself fails the frame is simply not in the ring
- * (fft3d_plane_has reports it). Ordering: the upload stream first waits for
- * the plane's earlier calls; a later process / forward_process whose window
+ * (fft3d_plane_has reports it). Ordering: the upload stream also waits for
+ * the plane's previous call; a later process / forward_process whose window
  * holds fn waits for the prefetch on its own stream, and a forward that
- * rewrites fn or reuses a pooled spectrum waits for the latest prefetch.
- * Host buffers only. */
-int fft3d_plane_prefetch(fft3d_plane* p, int fn, const void* src, size_t bytes);
+ * rewrites fn or reuses a pooled spectrum waits for the latest prefetch. */
+int fft3d_plane_prefetch(fft3d_plane* p, int fn, const void* src, size_t bytes,
+                         int src_on_device);
 
 /* Drops ring entries outside [lo, hi]; a negative bound is open
  * (GpuRef::pruneCache, src/cl/gpu.cpp:507-517). Retired spectra go to a

@@ -348,7 +348,13 @@ def test_emu_prefetch_calls(emu_lib):
     pl.process([2, 3, 5], out)
     ref.forward(5, planes[0][5])
     assert np.array_equal(pl.read_spectrum(5), ref.read_spectrum(5))
-    rc = pl.lib.fft3d_plane_prefetch(pl.h, 1, planes[0][1].ctypes.data, 10)
+    # a device source (here the emulation's memory): the analysis reads it
+    src4 = np.ascontiguousarray(planes[0][4])
+    assert pl.lib.fft3d_plane_prefetch(pl.h, 4, src4.ctypes.data, pl.nbytes, 1) == 0
+    pl.process([2, 3, 5], out)
+    ref.forward(4, planes[0][4])
+    assert np.array_equal(pl.read_spectrum(4), ref.read_spectrum(4))
+    rc = pl.lib.fft3d_plane_prefetch(pl.h, 1, planes[0][1].ctypes.data, 10, 0)
     assert rc == -1 and "too small" in pl.error
     with pytest.raises(ValueError):
         class Dev:

@@ -135,24 +135,40 @@ def test_device_resident_path_equals_host_path(gpu):
     assert np.array_equal(dev_out.cpu().numpy().view(np.uint16), host)
 
 
-@pytest.mark.parametrize("pinned", [False, True])
-def test_prefetch_equals_plain(gpu, pinned):
+@pytest.mark.parametrize("where", ["pageable", "pinned", "device"])
+def test_prefetch_equals_plain(gpu, where):
     """Sequential requests with the next new frame prefetched on the upload
-    stream give the same frames as plain requests (4K, host buffers)."""
+    stream give the same frames as plain requests (4K; host buffers, or
+    device buffers on a caller stream)."""
     W, H, n = 3840, 2160, 8
     frames = make_plane_sequence(12, W, H, n, "u10")
-    src = torch.from_numpy(frames).pin_memory() if pinned else frames
+    dev = where == "device"
+    if where == "pinned":
+        src = torch.from_numpy(frames).pin_memory()
+    elif dev:
+        src = torch.from_numpy(frames).cuda()
+    else:
+        src = frames
+    ts = torch.cuda.Stream()
     outs = []
     for prefetch in (True, False):
         inst = Instance(1, W, H, 10, 0, n, bt=4, sharpen=0.4, lib=gpu, prefetch=prefetch)
-        if pinned:
+        if where == "pinned":
             out = torch.zeros((n, H, W), dtype=torch.int16).pin_memory()
+        elif dev:
+            out = torch.zeros((n, H, W), dtype=torch.int16, device="cuda")
         else:
             out = np.zeros_like(frames)
+        torch.cuda.synchronize()
         for k in range(n):
-            inst.get_frame(k, [lambda fn: src[fn]], [out[k]])
+            inst.get_frame(k, [lambda fn: src[fn]], [out[k]], on_device=dev,
+                           stream=ts.cuda_stream if dev else None)
+        torch.cuda.synchronize()
         inst.close()
-        outs.append(out.numpy().view(np.uint16) if pinned else out)
+        if dev:
+            outs.append(out.cpu().numpy().view(np.uint16))
+        else:
+            outs.append(out.numpy().view(np.uint16) if where == "pinned" else out)
     assert outs[1].any()
     assert np.array_equal(outs[0], outs[1])
 
@@ -174,11 +190,21 @@ def test_prefetched_frame_used_at_once(gpu, dst_on_device):
         outs = [torch.zeros((H, W), dtype=torch.int16, device="cuda") if dst_on_device
                 else np.zeros((H, W), np.uint16) for _ in range(3)]
         torch.cuda.synchronize()
+        keep = []
         for k in range(4):
             pl.forward(k, frames[k], False, stream)
         for i, k in enumerate((4, 5)):
             if pre:
-                pl.prefetch(k, frames[k])   # carried out by this process call
+                if dst_on_device:
+                    # a device source written on the caller stream just before
+                    src = torch.empty((H, W), dtype=torch.int16, device="cuda")
+                    host = torch.from_numpy(frames[k].view(np.int16))
+                    with torch.cuda.stream(ts):
+                        src.copy_(host)
+                    keep.append(src)
+                    pl.prefetch(k, src, True)
+                else:
+                    pl.prefetch(k, frames[k])   # carried out by this process call
             else:
                 pl.forward(k, frames[k], False, stream)
             pl.process([k - 4, k - 3, k - 2, k - 1], outs[i], dst_on_device, stream)
