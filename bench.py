@@ -14,8 +14,8 @@ Prints one JSON line. Per-kernel device times come from HIP events recorded on
 the launch stream around every kernel of every PROFILE_EVERY-th step of the
 timed region (bracketing every kernel of every step adds about 9 % to the
 step time; the sampled events stay inside the timed region); `roofline`
-prices the dominant kernel's algorithmic bytes per launch (per-unit figures
-in DESIGN.md) against the 8 TB/s HBM peak. `cpu_baseline` times the
+prices the dominant kernel's algorithmic bytes per launch (kernel_bytes:
+per-unit figures times the units of one launch) against the 8 TB/s HBM peak. `cpu_baseline` times the
 reference's CPU path (its own sources built into oracle/_ref when present,
 else the oracle restatement) single-threaded on a bounded 3-frame sample of
 the same clip.
@@ -253,8 +253,8 @@ def main():
                          "reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        sys.stderr.write("bench.py measures one GPU only: this build has no multi-GPU path "
-                         "(streams shard across GPUs without communication; see DESIGN.md)\n")
+        sys.stderr.write("bench.py measures one GPU only: there is no multi-GPU path "
+                         "(whole streams would go to separate GPUs, with no communication)\n")
         sys.exit(2)
 
     import numpy as np
