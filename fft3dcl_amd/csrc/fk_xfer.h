@@ -1,16 +1,16 @@
-// Host <-> device transfers of planes held in pageable host memory.
+// Host <-> device transfers of planes.
 //
-// A pageable buffer cannot be read or written by the DMA engine directly, so
-// the plane goes through a few pinned chunks owned by the plane object. Host
+// A pageable buffer cannot be read or written by the device directly, so the
+// plane goes through a few pinned chunks owned by the plane object. Host
 // threads copy chunk i between the caller's buffer and a pinned chunk while
-// the DMA engine moves its neighbours:
+// the device moves its neighbours:
 //   upload   : copy chunk i into pinned slot i % nbuf, queue its DMA, go on
 //              (a slot is refilled once the DMA that read it has finished);
-//   download : queue the DMAs of the first nbuf chunks, then for each chunk
-//              wait for its DMA, copy it out, and queue the DMA of the chunk
-//              nbuf ahead into the slot just emptied.
-// Pinned (page-locked) and device buffers go to the DMA engine as they are.
-// Device -> pinned host copies run as a kernel (k_copy_out) on the caller's
+//   download : queue the device-side copies of the first nbuf chunks, then
+//              for each chunk wait for its copy, copy it out, and queue the
+//              chunk nbuf ahead into the slot just emptied.
+// Page-locked and device buffers are handed over as they are. Device ->
+// page-locked host copies run as a kernel (k_copy_out) on the caller's
 // stream rather than on the DMA engine: on this platform a host-to-device DMA
 // and a device-to-host DMA do not overlap, but a copy kernel writing host
 // memory overlaps a host-to-device DMA (fft3d_plane_prefetch relies on it).
