@@ -135,11 +135,13 @@ def test_device_resident_path_equals_host_path(gpu):
     assert np.array_equal(dev_out.cpu().numpy().view(np.uint16), host)
 
 
-@pytest.mark.parametrize("where", ["pageable", "pinned", "device"])
-def test_prefetch_equals_plain(gpu, where):
+@pytest.mark.parametrize("where,bt,half", [("pageable", 4, False), ("pinned", 4, False),
+                                           ("device", 4, False), ("pageable", 3, False),
+                                           ("pinned", 2, True)])
+def test_prefetch_equals_plain(gpu, where, bt, half):
     """Sequential requests with the next new frame prefetched on the upload
     stream give the same frames as plain requests (4K; host buffers, or
-    device buffers on a caller stream)."""
+    device buffers on a caller stream; FP32 or FP16 ring)."""
     W, H, n = 3840, 2160, 8
     frames = make_plane_sequence(12, W, H, n, "u10")
     dev = where == "device"
@@ -152,7 +154,8 @@ def test_prefetch_equals_plain(gpu, where):
     ts = torch.cuda.Stream()
     outs = []
     for prefetch in (True, False):
-        inst = Instance(1, W, H, 10, 0, n, bt=4, sharpen=0.4, lib=gpu, prefetch=prefetch)
+        inst = Instance(1, W, H, 10, 0, n, bt=bt, sharpen=0.4, lib=gpu, prefetch=prefetch,
+                        half=half)
         if where == "pinned":
             out = torch.zeros((n, H, W), dtype=torch.int16).pin_memory()
         elif dev:
