@@ -208,8 +208,9 @@ def served_sample(seconds=0.5, nstreams=32, uniq=3):
     """The served form (32 seeded streams of mixed resolution, bit depth, bt
     and block size, Poisson arrivals at 24-60 frames/s each over `seconds`):
     p50/p99 latency at that offered load, and the capacity of the same
-    requests issued all at once. Frames are seeded noise made on the GPU
-    (`uniq` per stream, cycled): the filter's cost does not depend on them."""
+    requests issued all at once, one call per request and batched through
+    fft3d_step. Frames are seeded noise made on the GPU (`uniq` per stream,
+    cycled): the filter's cost does not depend on them."""
     import torch
     from fft3dcl_amd.serve import Server, make_specs, summarize
     specs = make_specs(SEED, nstreams, duration=seconds)
@@ -224,13 +225,18 @@ def served_sample(seconds=0.5, nstreams=32, uniq=3):
     budget = 64 << 30
     Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=False)  # warm-up
     cap = Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=False)
+    capb = Server(specs, sources, lambda sid, n: out[sid], budget, batched=True).run(paced=False)
     paced = Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=True)
+    pacedb = Server(specs, sources, lambda sid, n: out[sid], budget,
+                    batched=True).run(paced=True)
     offered = sum(sp.nframes for sp in specs) / seconds
     res = {"offered_frames_per_s": round(offered, 1), "at_offered_load": summarize(specs, paced),
-           "capacity": summarize(specs, cap),
+           "at_offered_load_batched": summarize(specs, pacedb),
+           "capacity": summarize(specs, cap), "capacity_batched": summarize(specs, capb),
            "mix": "seeded: resolutions 640x360..3840x2160, 8/10-bit, bt 0/2/3/4, bw 16/32/64"}
-    res["capacity"].pop("p50_ms")
-    res["capacity"].pop("p99_ms")
+    for k in ("capacity", "capacity_batched"):
+        res[k].pop("p50_ms")
+        res[k].pop("p99_ms")
     return res
 
 

@@ -52,13 +52,23 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_int), ("total_ms", C.c_double)]
 
 
+SOURCE_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_int)
+
+
+class WorkItem(C.Structure):
+    _fields_ = [("plane", C.c_void_p), ("frame", C.c_int), ("num_frames", C.c_int),
+                ("source", SOURCE_FN), ("user", C.c_void_p), ("src_on_device", C.c_int),
+                ("dst", C.c_void_p), ("bytes", C.c_size_t), ("dst_on_device", C.c_int),
+                ("stream", C.c_void_p), ("status", C.c_int)]
+
+
 EXPORTS = [
     "fft3d_plane_set_profiling", "fft3d_plane_get_profile", "fft3d_plane_set_option",
     "fft3d_params_default", "fft3d_dev_init", "fft3d_plane_create", "fft3d_plane_destroy",
     "fft3d_plane_error", "fft3d_plane_has", "fft3d_plane_forward", "fft3d_plane_process",
     "fft3d_plane_prune", "fft3d_plane_clear", "fft3d_live_instances", "fft3d_live_bytes",
     "fft3d_plane_get_info", "fft3d_plane_get_table", "fft3d_plane_read_spectrum",
-    "fft3d_plane_forward_process", "fft3d_plane_prefetch",
+    "fft3d_plane_forward_process", "fft3d_plane_prefetch", "fft3d_step",
 ]
 
 
@@ -98,6 +108,7 @@ def load(path: str | Path | None = None) -> C.CDLL:
     L.fft3d_plane_forward_process.argtypes = [vp, i, vp, sz, i, C.POINTER(C.c_int), i, vp, sz,
                                               i, vp]
     L.fft3d_plane_prefetch.argtypes = [vp, i, vp, sz, i]
+    L.fft3d_step.argtypes = [C.POINTER(WorkItem), i]
     L.fft3d_plane_prune.argtypes = [vp, i, i]
     L.fft3d_plane_prune.restype = None
     L.fft3d_plane_clear.argtypes = [vp]
@@ -280,3 +291,35 @@ def dev_init(device: int = 0, lib=None):
     rc = load(lib).fft3d_dev_init(device, err, 256)
     if rc != FFT3D_OK:
         raise FFT3DError(rc, err.value.decode())
+
+
+class StepRequest:
+    """One work item of `step`: output frame `frame` of `plane` into `dst`;
+    `source(fn)` gives the plane of any frame the window needs."""
+
+    def __init__(self, plane: Plane, frame: int, num_frames: int, source, dst,
+                 on_device: bool = False, stream: int | None = None):
+        self.plane, self.frame, self.num_frames = plane, frame, num_frames
+        self.source, self.dst, self.on_device, self.stream = source, dst, on_device, stream
+
+
+def step(requests: list, lib=None) -> list[int]:
+    """Runs the requests through one fft3d_step call (the serving path's
+    batched host step) and returns each item's status (0 = FFT3D_OK). The
+    buffers the sources hand out are kept alive until the call returns."""
+    L = load(lib)
+    keep, cbs = [], []
+    items = (WorkItem * max(1, len(requests)))()
+    for k, r in enumerate(requests):
+        def cb(_user, fn, src=r.source):
+            buf = src(fn)
+            if buf is None:
+                return None
+            keep.append(buf)
+            return _ptr(buf)
+        c = SOURCE_FN(cb)
+        cbs.append(c)
+        items[k] = WorkItem(r.plane.h, r.frame, r.num_frames, c, None, int(r.on_device),
+                            _ptr(r.dst), r.plane.nbytes, int(r.on_device), r.stream, 0)
+    L.fft3d_step(items, len(requests))
+    return [items[k].status for k in range(len(requests))]

@@ -917,6 +917,53 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
   return rc;
 }
 
+int fft3d_step(fft3d_work_item* items, int n) {
+  if (!items || n < 0) return FFT3D_EINVAL;
+  int first = FFT3D_OK;
+  for (int k = 0; k < n; k++) {
+    fft3d_work_item& it = items[k];
+    fft3d_plane* p = it.plane;
+    int rc = FFT3D_OK;
+    if (!p || !it.source || !it.dst || it.num_frames <= 0 || it.frame < 0 ||
+        it.frame >= it.num_frames) {
+      rc = FFT3D_EINVAL;
+      if (p) p->fail(rc, "bad work item");
+    } else {
+      const int bm = p->p.bm, bt = p->p.bt, cur = p->p.current, nf = it.num_frames;
+      auto clampf = [nf](int f) { return f < 0 ? 0 : (f > nf - 1 ? nf - 1 : f); };
+      int window[4], missing[4], nwin = bm >= 2 ? bt : 1, nmiss = 0;
+      for (int i = 0; i < nwin; i++) {
+        const int fn = clampf(bm >= 2 ? it.frame - cur + i : it.frame);
+        window[i] = fn;
+        bool seen = false;
+        for (int j = 0; j < nmiss; j++) seen = seen || missing[j] == fn;
+        if (!seen && !p->ring.count(fn)) missing[nmiss++] = fn;
+      }
+      const void* src[4] = {};
+      for (int j = 0; j < nmiss && rc == FFT3D_OK; j++)
+        if (!(src[j] = it.source(it.user, missing[j])))
+          rc = p->fail(FFT3D_EINVAL, "work item source gave no plane");
+      // the missing frames but the last through forward, the last with the
+      // filter pass (forward_process)
+      for (int j = 0; j + 1 < nmiss && rc == FFT3D_OK; j++)
+        rc = fft3d_plane_forward(p, missing[j], src[j], it.bytes, it.src_on_device, it.stream);
+      if (rc == FFT3D_OK)
+        rc = nmiss ? fft3d_plane_forward_process(p, missing[nmiss - 1], src[nmiss - 1], it.bytes,
+                                                 it.src_on_device, window, nwin, it.dst,
+                                                 it.bytes, it.dst_on_device, it.stream)
+                   : fft3d_plane_process(p, window, nwin, it.dst, it.bytes, it.dst_on_device,
+                                         it.stream);
+      if (rc == FFT3D_OK) {
+        if (bm >= 2) fft3d_plane_prune(p, it.frame - cur - 1, it.frame - cur + bt);
+        else if (bm != 0) fft3d_plane_clear(p);
+      }
+    }
+    it.status = rc;
+    if (rc != FFT3D_OK && first == FFT3D_OK) first = rc;
+  }
+  return first;
+}
+
 void fft3d_plane_prune(fft3d_plane* p, int lo, int hi) {
   if (!p) return;
   if (p->pendFn >= 0 && !((lo < 0 || p->pendFn >= lo) && (hi < 0 || p->pendFn <= hi)))

@@ -18,6 +18,11 @@ the start of the run.
 Without a GPU (tests against the CPU emulation) the same loop runs with host
 buffers and no streams; the results must then equal those of each stream
 processed alone.
+
+With batched=True, the requests that one pass of the loop issues, over all
+admitted streams, go to the library in one fft3d_step call (each item on its
+stream's HIP stream); their completion events are recorded per stream after
+the call.
 """
 from __future__ import annotations
 
@@ -27,6 +32,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import capi
 from .filter import GRAY, Instance
 
 __all__ = ["StreamSpec", "make_specs", "footprint", "Server"]
@@ -98,7 +104,9 @@ class Server:
     sid (a device tensor on a GPU run, a numpy array otherwise); `outputs`
     (sid, n) -> buffer receives the filtered frame."""
 
-    def __init__(self, specs, sources, outputs, budget: int, lib=None, on_device=True):
+    def __init__(self, specs, sources, outputs, budget: int, lib=None, on_device=True,
+                 batched: bool = False):
+        self.batched = batched
         self.specs = {s.sid: s for s in specs}
         self.sources = sources
         self.outputs = outputs
@@ -161,26 +169,38 @@ class Server:
                 pending[reqs[i][1]].append((reqs[i][2], reqs[i][0]))
                 i += 1
             issued = False
+            batch = []  # (sid, n, arrival, request) of this pass when batched
             for sid in list(admitted):
                 a = admitted[sid]
                 while pending[sid] and pending[sid][0][0] == a["next"]:
                     n, arr = pending[sid].pop(0)
-                    srcs = [lambda fn, s=sid: self.sources[s](fn)]
                     h = a["stream"].cuda_stream if dev else None
-                    a["inst"].get_frame(n, srcs, [self.outputs(sid, n)], on_device=dev,
-                                        stream=h)
-                    ev = None
-                    if dev:
-                        ev = torch.cuda.Event(enable_timing=True)
-                        ev.record(a["stream"])
-                    done_events.append((sid, n, arr, ev))
+                    if self.batched:
+                        batch.append((sid, n, arr, capi.StepRequest(
+                            a["inst"].objs[0], n, self.specs[sid].nframes,
+                            lambda fn, s=sid: self.sources[s](fn), self.outputs(sid, n),
+                            on_device=dev, stream=h)))
+                    else:
+                        srcs = [lambda fn, s=sid: self.sources[s](fn)]
+                        a["inst"].get_frame(n, srcs, [self.outputs(sid, n)], on_device=dev,
+                                            stream=h)
+                        ev = None
+                        if dev:
+                            ev = torch.cuda.Event(enable_timing=True)
+                            ev.record(a["stream"])
+                        done_events.append((sid, n, arr, ev))
                     a["next"] += 1
                     issued = True
+                if self.batched and batch and a["next"] == self.specs[sid].nframes and waiting:
+                    self._flush(batch, admitted, done_events, dev)
+                    batch = []
                 if a["next"] == self.specs[sid].nframes and waiting:
                     # free its memory only when another stream waits for it:
                     # retiring waits for the stream's GPU work
                     retire(sid)
                     admit()
+            if batch:
+                self._flush(batch, admitted, done_events, dev)
             if not issued and i < len(reqs) and paced:
                 time.sleep(max(0.0, min(0.0005, reqs[i][0] - (time.perf_counter() - t0))))
         for sid in list(admitted):
@@ -196,6 +216,22 @@ class Server:
             span = 1000.0 * (time.perf_counter() - t0)
         return dict(latencies_ms=lat, span_ms=span, requests=len(done_events),
                     max_admitted=self.max_admitted)
+
+
+    def _flush(self, batch, admitted, done_events, dev):
+        """One fft3d_step call for the batched requests, then a completion
+        event per stream that had items in it."""
+        import torch
+        st = capi.step([r for *_, r in batch], lib=self.lib)
+        bad = [(b[0], b[1], c) for b, c in zip(batch, st) if c != 0]
+        if bad:
+            raise capi.FFT3DError(bad[0][2], f"fft3d_step failed for (stream, frame) {bad[:3]}")
+        evs = {}
+        for sid, n, arr, _ in batch:
+            if dev and sid not in evs:
+                evs[sid] = torch.cuda.Event(enable_timing=True)
+                evs[sid].record(admitted[sid]["stream"])
+            done_events.append((sid, n, arr, evs.get(sid)))
 
 
 def summarize(specs, res) -> dict:

@@ -158,6 +158,40 @@ void fft3d_plane_prune(fft3d_plane* p, int lo, int hi);
  * src/fft3dcl.cpp:465, which in the reference keeps everything). */
 void fft3d_plane_clear(fft3d_plane* p);
 
+/* The serving path's batched step (no counterpart call in the reference: it
+ * runs the reference's per-frame host step, src/fft3dcl.cpp:319-331 and
+ * :396-467, for many planes at once). One work item is one output frame of
+ * one plane. For each item, in order, the library:
+ *   - builds the clamped temporal window of `frame` (bt >= 2: frames
+ *     frame - current + i, i < bt, clamped to [0, num_frames - 1]; else the
+ *     frame itself);
+ *   - forwards the window frames missing from the ring, asking `source` for
+ *     each one's plane (`bytes` long; on the device if src_on_device);
+ *   - processes the window into `dst` on `stream` (null: the plane's own);
+ *   - prunes the ring as the reference does (bt >= 2: keep
+ *     [frame - current - 1, frame - current + bt]; bt 1 and -1: clear).
+ * Items of one plane must come in the order their frames are wanted (Kalman
+ * frames in order). With device outputs nothing blocks: the items of
+ * different planes overlap on their streams. A host output is complete when
+ * the call returns. `status` receives each item's result. The return value
+ * is FFT3D_OK, or the code of the first failing item (the plane's error
+ * string has the reason); later items still run. */
+typedef const void* (*fft3d_source_fn)(void* user, int frame);
+typedef struct fft3d_work_item {
+  fft3d_plane* plane;
+  int frame;
+  int num_frames;
+  fft3d_source_fn source;
+  void* user;
+  int src_on_device;
+  void* dst;
+  size_t bytes;
+  int dst_on_device;
+  void* stream;
+  int status;
+} fft3d_work_item;
+int fft3d_step(fft3d_work_item* items, int n);
+
 /* Process-wide live plane objects and device bytes
  * (GpuRef::liveInstances / liveBytes, src/cl/gpu.h:27-28). */
 int fft3d_live_instances(void);

@@ -364,6 +364,59 @@ def test_emu_prefetch_calls(emu_lib):
     ref.close()
 
 
+@pytest.mark.parametrize("kw,order", [
+    (dict(bt=4, sharpen=0.4), [0, 1, 2, 3, 4, 5]),
+    (dict(bt=3), [0, 1, 5, 2, 3, 4]),                     # a seek: the window is rebuilt
+    (dict(bt=0, sigma2=3.0), [0, 1, 2, 3, 4, 5]),         # Kalman, in order
+    (dict(bt=1), [3, 1, 2]),
+    (dict(bt=-1, sharpen=0.5), [2, 0]),
+])
+def test_emu_step_equals_host_step(emu_lib, kw, order):
+    """fft3d_step (window, forwards of missing frames, process, prune in the
+    library) gives the frames of the Python host step, with the items of two
+    planes interleaved in each call."""
+    from fft3dcl_amd.filter import Instance
+    W, H, n = 120, 80, 6
+    clips = [make_clip(20 + c, W, H, n, "u8")[0] for c in range(2)]
+    want = []
+    for c in range(2):
+        inst = Instance(1, W, H, 8, 0, n, lib=emu_lib, **kw)
+        out = np.zeros_like(clips[c])
+        for f in order:
+            inst.get_frame(f, [lambda fn, a=clips[c]: np.ascontiguousarray(a[fn])], [out[f]])
+        inst.close()
+        want.append(out)
+    params = capi.make_params(**kw)
+    planes = [capi.Plane(params, W, H, 8, lib=emu_lib) for _ in range(2)]
+    got = [np.zeros_like(a) for a in clips]
+    for i in range(0, len(order), 2):
+        reqs = [capi.StepRequest(planes[c], f, n, lambda fn, a=clips[c]: np.ascontiguousarray(a[fn]),
+                                 got[c][f])
+                for f in order[i:i + 2] for c in range(2)]
+        assert capi.step(reqs, lib=emu_lib) == [0] * len(reqs)
+    for c in range(2):
+        assert np.array_equal(got[c][order], want[c][order])
+        if kw["bt"] >= 2:
+            assert planes[c].info.cached_frames <= kw["bt"] + 2
+        elif kw["bt"] != 0:
+            assert planes[c].info.cached_frames == 0
+        planes[c].close()
+
+
+def test_emu_step_errors(emu_lib):
+    clip = make_clip(22, 96, 64, 3, "u8")[0]
+    pl = capi.Plane(capi.make_params(bt=3), 96, 64, 8, lib=emu_lib)
+    out = np.zeros_like(clip)
+    src = lambda fn: np.ascontiguousarray(clip[fn])
+    reqs = [capi.StepRequest(pl, 0, 3, lambda fn: None, out[0]),   # no plane from the source
+            capi.StepRequest(pl, 5, 3, src, out[1]),               # frame past the clip
+            capi.StepRequest(pl, 1, 3, src, out[1])]               # fine
+    st = capi.step(reqs, lib=emu_lib)
+    assert st[0] == -1 and st[1] == -1 and st[2] == 0
+    assert out[1].any()
+    pl.close()
+
+
 def test_emu_staging_options(emu_lib):
     pl = capi.Plane(capi.make_params(bt=3), 96, 64, 8, lib=emu_lib)
     for opt, bad in ((capi.Plane.OPT_HOST_STAGING, 2), (capi.Plane.OPT_STAGING_CHUNK, 300),

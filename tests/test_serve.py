@@ -40,14 +40,15 @@ def _alone(sp, clip, lib, on_device):
     return out
 
 
-def _mix(specs, clips, lib, on_device, budget, paced):
+def _mix(specs, clips, lib, on_device, budget, paced, batched=False):
     if on_device:
         import torch
         outs = {sp.sid: torch.zeros_like(clips[sp.sid]) for sp in specs}
     else:
         outs = {sp.sid: np.zeros_like(clips[sp.sid]) for sp in specs}
     srv = Server(specs, {sid: (lambda fn, c=c: c[fn]) for sid, c in clips.items()},
-                 lambda sid, n: outs[sid][n], budget, lib=lib, on_device=on_device)
+                 lambda sid, n: outs[sid][n], budget, lib=lib, on_device=on_device,
+                 batched=batched)
     res = srv.run(paced=paced)
     if on_device:
         import torch
@@ -67,12 +68,13 @@ def test_specs_are_seeded_and_mixed():
     assert all(np.all(np.diff(s.arrivals) > 0) for s in a)
 
 
-def test_emu_mix_equals_alone_with_staged_admission(emu_lib):
+@pytest.mark.parametrize("batched", [False, True])
+def test_emu_mix_equals_alone_with_staged_admission(emu_lib, batched):
     specs = make_specs(11, 6, duration=0.2, resolutions=[(96, 64), (128, 80), (160, 96)],
                        max_frames=4)
     clips = _clips(specs, False)
     budget = max(footprint(s) for s in specs) * 2  # about two streams at a time
-    outs, res = _mix(specs, clips, emu_lib, False, budget, paced=False)
+    outs, res = _mix(specs, clips, emu_lib, False, budget, paced=False, batched=batched)
     assert res["requests"] == sum(s.nframes for s in specs)
     assert 1 <= res["max_admitted"] < len(specs)
     for sp in specs:
@@ -80,11 +82,12 @@ def test_emu_mix_equals_alone_with_staged_admission(emu_lib):
 
 
 @pytest.mark.gpu
-def test_gpu_mix_equals_alone(gpu):
+@pytest.mark.parametrize("batched", [False, True])
+def test_gpu_mix_equals_alone(gpu, batched):
     specs = make_specs(12, 8, duration=0.25, resolutions=[(640, 360), (1280, 720)],
                        max_frames=6)
     clips = _clips(specs, True)
-    outs, res = _mix(specs, clips, gpu, True, 64 << 30, paced=True)
+    outs, res = _mix(specs, clips, gpu, True, 64 << 30, paced=True, batched=batched)
     s = summarize(specs, res)
     assert res["requests"] == sum(sp.nframes for sp in specs)
     assert s["p99_ms"] >= s["p50_ms"] > 0
