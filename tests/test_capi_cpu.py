@@ -273,7 +273,9 @@ def test_emu_staged_host_transfers(emu_lib, chunk, fuse):
     ragged last one, or one chunk split over the copy threads: the plane is
     above 256 KiB) give the same frames as direct copies."""
     from fft3dcl_amd.filter import Instance
-    W, H = 400, 330  # 264,000 bytes of u16
+    # 264,000 bytes of u16 for the one-chunk case (split over the copy
+    # threads above 256 KiB); smaller planes for the many-chunk cases
+    W, H = (400, 330) if chunk == 0 else (200, 120)
     planes = make_clip(16, W, H, 3, "u10")
     outs = []
     for staging, ch in ((0, 0), (1, chunk)):
