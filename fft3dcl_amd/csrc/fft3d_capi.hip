@@ -152,7 +152,7 @@ struct fft3d_plane {
   bool fuseOn = false;       // option: forward_process analyses inside the sweep
   bool half = false;         // option: spectra stored as FP16 pairs
   float* dPart = nullptr;    // 4 partial planes
-  int2* dSeams = nullptr;    // border tiles of the current band layout
+  fk::SeamTile* dSeams = nullptr;  // border tiles of the current band layout
   int nseams = 0, seamR = 0;
   fk::SweepArgs sw{};
 
@@ -357,20 +357,42 @@ struct fft3d_plane {
       for (int t = st * nbw; t <= st * nbw + nlive - 2; t++)
         if (t >= 0 && t < ntx) colIn[t] = 1;
     }
-    std::vector<int2> seams;
+    // each border tile with the partial planes of the workgroups that touch
+    // it, in the order the combine adds them (bands, then strips)
+    std::vector<fk::SeamTile> seams;
     for (int u = 0; u < nty; u++)
-      for (int t = 0; t < ntx; t++)
-        if (!rowIn[u] || !colIn[t]) seams.push_back(make_int2(t, u));
+      for (int t = 0; t < ntx; t++) {
+        if (rowIn[u] && colIn[t]) continue;
+        // band i covers tile rows [i*R - 1, min(i*R + R, nrows) - 1]
+        int bands[2], nbd = 0;
+        for (int i = u / R; i <= (u + 1) / R && nbd < 2; i++)
+          if (i < nbands && i * R - 1 <= u && u <= std::min(i * R + R, nrows) - 1)
+            if (nbd == 0 || bands[nbd - 1] != i) bands[nbd++] = i;
+        // strip st covers tile columns [st*nbw - 1, st*nbw + nlive - 1]
+        int strips[2], nst = 0;
+        for (int st = t / nbw; st <= (t + 1) / nbw && nst < 2; st++) {
+          if (st >= nstrips) continue;
+          const int nlive = std::min(nbw, ncols - st * nbw);
+          if (st * nbw - 1 <= t && t <= st * nbw + nlive - 1)
+            if (nst == 0 || strips[nst - 1] != st) strips[nst++] = st;
+        }
+        fk::SeamTile tile{t, u, 0, 0};
+        for (int a = 0; a < nbd; a++)
+          for (int c = 0; c < nst; c++)
+            tile.planes |= ((bands[a] & 1) * 2 + (strips[c] & 1)) << (2 * tile.np++);
+        seams.push_back(tile);
+      }
     if (dSeams) {
       (void)hipFree(dSeams);
-      bytes -= (size_t)nseams * sizeof(int2);
-      g_bytes -= (size_t)nseams * sizeof(int2);
+      bytes -= (size_t)nseams * sizeof(fk::SeamTile);
+      g_bytes -= (size_t)nseams * sizeof(fk::SeamTile);
       dSeams = nullptr;
     }
     nseams = (int)seams.size();
-    if (nseams && (!alloc(reinterpret_cast<void**>(&dSeams), seams.size() * sizeof(int2)) ||
-                   hipMemcpy(dSeams, seams.data(), seams.size() * sizeof(int2),
-                             hipMemcpyHostToDevice) != hipSuccess)) {
+    const size_t seamBytes = seams.size() * sizeof(fk::SeamTile);
+    if (nseams && (!alloc(reinterpret_cast<void**>(&dSeams), seamBytes) ||
+                   hipMemcpy(dSeams, seams.data(), seamBytes, hipMemcpyHostToDevice) !=
+                       hipSuccess)) {
       (void)hipGetLastError();
       return false;
     }
@@ -476,7 +498,7 @@ int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipS
     if (a.nseams) {
       CombineFn kc = &fk::k_combine_seams;
       t0 = prof_begin(s);
-      FK_LAUNCH(kc, dim3((a.nseams + 3) / 4), dim3(256), s, args, a, ks.nbw);
+      FK_LAUNCH(kc, dim3((a.nseams + 3) / 4), dim3(256), s, args, a);
       prof_end(t0, K_COMBINE, s);
       if (!launched("seam combine")) return FFT3D_ELAUNCH;
     }

@@ -12,7 +12,7 @@ using AnalysisFn = void (*)(PlaneArgs, const void*, c32*);
 using FilterFn = void (*)(PlaneArgs, WindowArgs, float4*, int, float*);
 using SynthFn = void (*)(PlaneArgs, const float*, void*);
 using SweepFn = void (*)(PlaneArgs, WindowArgs, float4*, int, SweepArgs);
-using CombineFn = void (*)(PlaneArgs, SweepArgs, int);
+using CombineFn = void (*)(PlaneArgs, SweepArgs);
 
 struct KernelSet {
   AnalysisFn analysis = nullptr;

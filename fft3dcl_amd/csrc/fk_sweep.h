@@ -28,6 +28,14 @@
 
 namespace fk {
 
+// A border tile and the partial planes that hold its contributions, in the
+// order they are added (bands, then strips): plane k is (planes >> 2k) & 3.
+struct SeamTile {
+  int x, y;    // tile column, tile row
+  int planes;  // packed plane indices
+  int np;      // 1..4
+};
+
 struct SweepArgs {
   int R;             // block rows per band
   int nstrips, nbands;
@@ -35,7 +43,7 @@ struct SweepArgs {
   float* part;       // 4 partial planes of H rows of ppitch floats
   int ppitch;        // row pitch of the partial planes: W rounded up to 4
   void* out;         // packed output plane (u8/u16/f32)
-  const int2* seams; // border tiles (tile column, tile row)
+  const SeamTile* seams;  // border tiles with their contributing planes
   int nseams;
   // Fused analysis of one new frame (the FWD instance of the sweep): its
   // plane, the ring spectrum to write, and the window slots that hold it
@@ -334,28 +342,17 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
 // them, in (band, strip) order, adds chroma centring and packs. One wave per
 // tile (so the contributor search is uniform), four samples per lane when the
 // tile width is a multiple of 4.
-__global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A, int nbw) {
+__global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A) {
   const int lane = threadIdx.x & 63;
   const int si = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (si >= A.nseams) return;  // no barriers below
-  const int2 tile = A.seams[si];
+  const SeamTile tile = A.seams[si];
   const int sx = P.bw / 2, sy = P.bh / 2;
-  // bands touching tile row u: band i covers tile rows [i*R - 1, i*R + R - 1]
-  int bands[2], nb = 0;
-  for (int i = (tile.y) / A.R; i <= (tile.y + 1) / A.R && nb < 2; i++)
-    if (i < A.nbands && i * A.R - 1 <= tile.y && tile.y <= min(i * A.R + A.R, P.nrows) - 1)
-      if (nb == 0 || bands[nb - 1] != i) bands[nb++] = i;
-  int strips[2], ns = 0;
-  for (int s = tile.x / nbw; s <= (tile.x + 1) / nbw && ns < 2; s++) {
-    if (s >= A.nstrips) continue;
-    const int nlive = min(nbw, P.ncols - s * nbw);
-    if (s * nbw - 1 <= tile.x && tile.x <= s * nbw + nlive - 1)
-      if (ns == 0 || strips[ns - 1] != s) strips[ns++] = s;
-  }
   const size_t plane = (size_t)A.ppitch * P.H;
-  int planes[4], np = 0;
-  for (int a = 0; a < nb; a++)
-    for (int c = 0; c < ns; c++) planes[np++] = (bands[a] & 1) * 2 + (strips[c] & 1);
+  const int np = tile.np;
+  int planes[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) planes[k] = (tile.planes >> (2 * k)) & 3;
   if (sx % 4 == 0) {
     const int qpr = sx / 4;  // 4-sample groups per tile row
     for (int q = lane; q < qpr * sy; q += 64) {
