@@ -339,9 +339,9 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
 
 #ifdef FK_COMMON_KERNELS
 // Adds the partial sums of border tiles from every workgroup that touched
-// them, in (band, strip) order, adds chroma centring and packs. One wave per
-// tile (so the contributor search is uniform), four samples per lane when the
-// tile width is a multiple of 4.
+// them, in (band, strip) order (the planes listed with each tile), adds
+// chroma centring and packs. One wave per tile, four samples per lane when
+// the tile width is a multiple of 4.
 __global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A) {
   const int lane = threadIdx.x & 63;
   const int si = blockIdx.x * 4 + (threadIdx.x >> 6);
