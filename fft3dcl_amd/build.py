@@ -20,9 +20,11 @@ HIP_LIB = PKG / "libfft3d_hip.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for every HIP source of this package.
+# The max-ilp machine scheduler orders the latency-bound kernels better than
+# the default (sweep 166 -> 162 us at 4K 10-bit bt=4; fewer spilled VGPRs).
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang",
-             "-packed-fp32-ops"]
+             "-packed-fp32-ops", "-mllvm", "--amdgpu-sched-strategy=max-ilp"]
 # translation units: the C ABI, and the kernels of each block shape
 BLOCK_SIZES = (4, 8, 16, 32, 64)
 JOBS = min(8, os.cpu_count() or 1)
