@@ -391,3 +391,30 @@ def test_fp16_ring_vs_oracle(gpu, oracle, oracle64, name):
         d = check(kind, g, w)
         print(f"{name} FP16 ring: max |GPU - oracle| {d} LSB, "
               f"max |GPU - FP64 oracle| {lsb_diff(g, w64)} LSB")
+
+
+def test_step_host_buffers_equals_host_step(gpu):
+    """fft3d_step over two planes (bt=4 and Kalman) with pageable host buffers,
+    their items interleaved in each call, gives the frames of the Python host
+    step."""
+    W, H, n = 1280, 720, 6
+    clips = [make_plane_sequence(14 + c, W, H, n, "u10") for c in range(2)]
+    kws = [dict(bt=4, sharpen=0.4), dict(bt=0, sigma2=3.0)]
+    want = []
+    for c in range(2):
+        inst = Instance(1, W, H, 10, 0, n, lib=gpu, **kws[c])
+        out = np.zeros_like(clips[c])
+        for f in range(n):
+            inst.get_frame(f, [lambda fn, a=clips[c]: a[fn]], [out[f]])
+        inst.close()
+        want.append(out)
+    planes = [capi.Plane(capi.make_params(**kws[c]), W, H, 10, lib=gpu) for c in range(2)]
+    got = [np.zeros_like(a) for a in clips]
+    for f in range(0, n, 2):
+        reqs = [capi.StepRequest(planes[c], g, n, lambda fn, a=clips[c]: a[fn], got[c][g])
+                for g in (f, f + 1) for c in range(2)]
+        assert capi.step(reqs, lib=gpu) == [0] * len(reqs)
+    for c in range(2):
+        planes[c].close()
+        assert want[c].any()
+        assert np.array_equal(got[c], want[c])
