@@ -976,8 +976,10 @@ int fft3d_step(fft3d_work_item* items, int n) {
                    : fft3d_plane_process(p, window, nwin, it.dst, it.bytes, it.dst_on_device,
                                          it.stream);
       if (rc == FFT3D_OK) {
+        // Kalman state lives in the plane's Kalman buffer: like the
+        // one-frame modes, bt=0 reuses no ring entry
         if (bm >= 2) fft3d_plane_prune(p, it.frame - cur - 1, it.frame - cur + bt);
-        else if (bm != 0) fft3d_plane_clear(p);
+        else fft3d_plane_clear(p);
       }
     }
     it.status = rc;

@@ -192,10 +192,14 @@ const VSFrame* getFrame(int n, int reason, void* instanceData, void** /*frameDat
       vsapi->freeFrame(src);
       return nullptr;
     }
-    // keep what the next requests reuse: the current window plus one frame of slack
+    // keep what the next requests reuse: the current window plus one frame of
+    // slack. One-frame modes reuse nothing; Kalman (bt=0) keeps its state in
+    // the plane's Kalman buffer, not in the ring, so its ring is dropped too
+    // (the reference never prunes it, src/fft3dcl.cpp:458-466, and grows by
+    // one spectrum per frame).
     if (d->bm >= 2)
       fft3d_plane_prune(g, n - d->current - 1, n - d->current + d->bt);
-    else if (d->bm != 0)
+    else
       fft3d_plane_clear(g);
   }
   if (frNext) vsapi->freeFrame(frNext);

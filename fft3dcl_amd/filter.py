@@ -114,7 +114,9 @@ class Instance:
                 obj.process(win, dst_planes[q], on_device, stream)
             if self.bm >= 2:
                 obj.prune(n - self.current - 1, n - self.current + self.bt)
-            elif self.bm != 0:
+            else:
+                # one-frame modes and Kalman (whose state is not in the ring)
+                # reuse no spectrum; the reference keeps bt=0 spectra forever
                 obj.clear()
 
     def close(self):

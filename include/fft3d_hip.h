@@ -168,8 +168,10 @@ void fft3d_plane_clear(fft3d_plane* p);
  *   - forwards the window frames missing from the ring, asking `source` for
  *     each one's plane (`bytes` long; on the device if src_on_device);
  *   - processes the window into `dst` on `stream` (null: the plane's own);
- *   - prunes the ring as the reference does (bt >= 2: keep
- *     [frame - current - 1, frame - current + bt]; bt 1 and -1: clear).
+ *   - prunes the ring (bt >= 2: keep [frame - current - 1,
+ *     frame - current + bt], as the reference does; bt 1, 0 and -1: clear.
+ *     Kalman state is not held in the ring; the reference never prunes a
+ *     bt=0 ring, src/fft3dcl.cpp:458-466).
  * Items of one plane must come in the order their frames are wanted (Kalman
  * frames in order). With device outputs nothing blocks: the items of
  * different planes overlap on their streams. A host output is complete when

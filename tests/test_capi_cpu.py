@@ -203,6 +203,50 @@ def test_emu_ring_prune_and_counts(emu_lib):
     assert capi.live_instances(emu_lib) == inst_before
 
 
+@pytest.mark.parametrize("bt", [0, 1, -1, 2, 4])
+def test_emu_ring_stays_bounded_over_long_runs(emu_lib, bt):
+    """Over 40 frames the ring never holds more than the window needs: bt >= 2
+    keeps at most bt + 2 spectra; bt 1, -1 and Kalman (bt=0, whose state is
+    not in the ring) hold none after a frame, and device memory stays flat.
+    The reference's bt=0 ring grows by one spectrum per frame
+    (src/fft3dcl.cpp:458-466)."""
+    from fft3dcl_amd.filter import Instance
+    n = 40
+    planes = make_clip(6, 64, 48, n, "u8", scene_cut=20)
+    inst = Instance(1, 64, 48, 8, 0, n, bt=bt, sigma2=3.0, lib=emu_lib)
+    srcs = [lambda fn: np.ascontiguousarray(planes[0][fn])]
+    out = np.zeros((48, 64), np.uint8)
+    dev = []
+    for k in range(n):
+        inst.get_frame(k, srcs, [out])
+        info = inst.objs[0].info
+        assert info.cached_frames <= (bt + 2 if bt >= 2 else 0), (k, info.cached_frames)
+        dev.append(info.device_bytes)
+    assert max(dev[4:]) == min(dev[4:]), dev
+    inst.close()
+
+
+def test_emu_step_kalman_ring_bounded(emu_lib):
+    """fft3d_step on a Kalman plane: no spectrum stays in the ring after an
+    item, over a long run, and the output equals the Python host step's."""
+    from fft3dcl_amd.filter import Instance
+    W, H, n = 96, 64, 30
+    clip = make_clip(23, W, H, n, "u8", scene_cut=15)[0]
+    src = lambda fn: np.ascontiguousarray(clip[fn])
+    inst = Instance(1, W, H, 8, 0, n, bt=0, lib=emu_lib)
+    want = np.zeros_like(clip)
+    for f in range(n):
+        inst.get_frame(f, [src], [want[f]])
+    inst.close()
+    pl = capi.Plane(capi.make_params(bt=0), W, H, 8, lib=emu_lib)
+    got = np.zeros_like(clip)
+    for f in range(n):
+        assert capi.step([capi.StepRequest(pl, f, n, src, got[f])], lib=emu_lib) == [0]
+        assert pl.info.cached_frames == 0
+    assert np.array_equal(got, want)
+    pl.close()
+
+
 @pytest.mark.parametrize("kind,kw,W,H,R", [
     ("u8", dict(bt=3), 600, 100, 2),                      # 3 strips x 4 bands
     ("u10", dict(bt=4, sharpen=0.4), 530, 90, 3),         # ragged last strip and band
@@ -400,7 +444,7 @@ def test_emu_step_equals_host_step(emu_lib, kw, order):
         assert np.array_equal(got[c][order], want[c][order])
         if kw["bt"] >= 2:
             assert planes[c].info.cached_frames <= kw["bt"] + 2
-        elif kw["bt"] != 0:
+        else:
             assert planes[c].info.cached_frames == 0
         planes[c].close()
 
