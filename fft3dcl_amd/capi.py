@@ -155,6 +155,34 @@ def _ptr(x) -> int:
     return int(x.data_ptr())
 
 
+def _buf(x, need: int, itemsize: int, on_device: bool, what: str) -> tuple[int, int]:
+    """(address, byte length) of a plane buffer handed to the C ABI, after
+    checking it: a C-contiguous numpy array or torch tensor whose elements
+    have the plane's sample size, at least `need` bytes long, on the device
+    exactly when `on_device`. Raises ValueError otherwise, so that a short or
+    strided buffer never reaches a copy or a kernel."""
+    if isinstance(x, np.ndarray):
+        if on_device:
+            raise ValueError(f"{what}: a numpy array is host memory, but on_device is set")
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError(f"{what}: the array is not C-contiguous")
+        size, nbytes = x.dtype.itemsize, x.nbytes
+    elif hasattr(x, "data_ptr") and hasattr(x, "element_size"):
+        if bool(getattr(x, "is_cuda", False)) != bool(on_device):
+            raise ValueError(f"{what}: on_device does not match the tensor's device")
+        if not x.is_contiguous():
+            raise ValueError(f"{what}: the tensor is not contiguous")
+        size, nbytes = x.element_size(), x.element_size() * x.numel()
+    else:
+        raise ValueError(f"{what}: expected a numpy array or a torch tensor, got "
+                         f"{type(x).__name__}")
+    if size != itemsize:
+        raise ValueError(f"{what}: {size}-byte elements, the plane has {itemsize}-byte samples")
+    if nbytes < need:
+        raise ValueError(f"{what}: {nbytes} bytes, the plane needs {need}")
+    return _ptr(x), nbytes
+
+
 class Plane:
     """One processed plane of one filter instance (the GpuRef counterpart)."""
 
@@ -204,15 +232,20 @@ class Plane:
     def has(self, fn: int) -> bool:
         return bool(self.lib.fft3d_plane_has(self.h, fn))
 
+    def buffer(self, x, on_device: bool, what: str) -> tuple[int, int]:
+        """(address, bytes) of a checked plane buffer (see _buf)."""
+        return _buf(x, self.nbytes, self.bps, on_device, what)
+
     def forward(self, fn: int, src, on_device: bool = False, stream: int | None = None):
-        self._check(self.lib.fft3d_plane_forward(self.h, fn, _ptr(src), self.nbytes,
-                                                 int(on_device), stream))
+        ptr, n = self.buffer(src, on_device, "forward source")
+        self._check(self.lib.fft3d_plane_forward(self.h, fn, ptr, n, int(on_device), stream))
 
     def process(self, window, dst, on_device: bool = False, stream: int | None = None):
         w = (C.c_int * len(window))(*window)
+        ptr, n = self.buffer(dst, on_device, "process output")
         try:
-            self._check(self.lib.fft3d_plane_process(self.h, w, len(window), _ptr(dst),
-                                                     self.nbytes, int(on_device), stream))
+            self._check(self.lib.fft3d_plane_process(self.h, w, len(window), ptr, n,
+                                                     int(on_device), stream))
         finally:
             self._pending = None
 
@@ -221,10 +254,12 @@ class Plane:
         """forward(fn, src) then process(window, dst) in one call; with
         OPT_FUSED_ANALYSIS set, frame fn is transformed inside the sweep."""
         w = (C.c_int * len(window))(*window)
+        sp, sn = self.buffer(src, src_on_device, "forward_process source")
+        dp, dn = self.buffer(dst, dst_on_device, "forward_process output")
         try:
             self._check(self.lib.fft3d_plane_forward_process(
-                self.h, fn, _ptr(src), self.nbytes, int(src_on_device), w, len(window),
-                _ptr(dst), self.nbytes, int(dst_on_device), stream))
+                self.h, fn, sp, sn, int(src_on_device), w, len(window), dp, dn,
+                int(dst_on_device), stream))
         finally:
             self._pending = None
 
@@ -235,8 +270,8 @@ class Plane:
         frame is used."""
         if bool(getattr(src, "is_cuda", False)) != bool(on_device):
             raise ValueError("prefetch: on_device does not match the buffer")
-        self._check(self.lib.fft3d_plane_prefetch(self.h, fn, _ptr(src), self.nbytes,
-                                                  int(on_device)))
+        ptr, n = self.buffer(src, on_device, "prefetch source")
+        self._check(self.lib.fft3d_plane_prefetch(self.h, fn, ptr, n, int(on_device)))
         self._pending = src
 
     def prune(self, lo: int, hi: int):
@@ -311,15 +346,20 @@ def step(requests: list, lib=None) -> list[int]:
     keep, cbs = [], []
     items = (WorkItem * max(1, len(requests)))()
     for k, r in enumerate(requests):
-        def cb(_user, fn, src=r.source):
+        def cb(_user, fn, src=r.source, pl=r.plane, dev=r.on_device):
             buf = src(fn)
             if buf is None:
                 return None
+            try:
+                ptr, _ = pl.buffer(buf, dev, "step source")
+            except ValueError:
+                return None  # the item fails with "source gave no plane"
             keep.append(buf)
-            return _ptr(buf)
+            return ptr
         c = SOURCE_FN(cb)
         cbs.append(c)
+        dptr, dn = r.plane.buffer(r.dst, r.on_device, "step output")
         items[k] = WorkItem(r.plane.h, r.frame, r.num_frames, c, None, int(r.on_device),
-                            _ptr(r.dst), r.plane.nbytes, int(r.on_device), r.stream, 0)
+                            dptr, dn, int(r.on_device), r.stream, 0)
     L.fft3d_step(items, len(requests))
     return [items[k].status for k in range(len(requests))]

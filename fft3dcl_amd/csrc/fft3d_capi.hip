@@ -132,7 +132,9 @@ struct fft3d_plane {
   hipStream_t up = nullptr;
   void* dStageUp = nullptr;
   fx::Staging stgUp;
-  hipEvent_t lastCall = nullptr;      // end of the latest other call
+  hipEvent_t lastCall = nullptr;      // end of the plane's latest call (any stream)
+  hipStream_t lastStream = nullptr;   // the stream that call used
+  bool anyCall = false;
   hipEvent_t upDone = nullptr;        // end of the latest prefetch
   std::map<int, hipEvent_t> upReady;  // prefetched spectrum -> end of its analysis
   std::vector<hipEvent_t> evSpare;
@@ -308,9 +310,24 @@ struct fft3d_plane {
     }
     return true;
   }
-  // marks the end of a call on stream s, for the next prefetch to wait on
+  // Calls of one plane share its staging buffer, ring, pool and partial
+  // planes. A call on another stream than the previous call first waits for
+  // that call's end, so callers may switch streams between calls.
+  bool order_call(hipStream_t s) {
+    if (!anyCall || s == lastStream) return true;
+    if (hipStreamWaitEvent(s, lastCall, 0) == hipSuccess) return true;
+    (void)hipGetLastError();
+    return false;
+  }
+  // marks the end of a call on stream s (for the next call on another
+  // stream, the next prefetch, and destroy)
   void end_call(hipStream_t s) {
-    if (up) (void)hipEventRecord(lastCall, s);
+    if (hipEventRecord(lastCall, s) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    lastStream = s;
+    anyCall = true;
   }
 
   void release(fk::c32* m) {
@@ -443,11 +460,9 @@ struct fft3d_plane {
     if (up) return true;
     bool ok = hipDeviceSynchronize() == hipSuccess &&
               hipStreamCreateWithFlags(&up, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&lastCall, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&upDone, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&callStart, hipEventDisableTiming) == hipSuccess;
-    ok = ok && hipEventRecord(lastCall, up) == hipSuccess &&
-         hipEventRecord(upDone, up) == hipSuccess;
+    ok = ok && hipEventRecord(upDone, up) == hipSuccess;
     if (!ok) {
       (void)hipGetLastError();
       if (up) (void)hipStreamDestroy(up);
@@ -634,6 +649,15 @@ int fft3d_dev_init(int device, char* err, size_t errlen) {
     set_err(err, errlen, "FFT3DCL: hipGetDeviceProperties failed");
     return FFT3D_EDEVICE;
   }
+#if !defined(FFT3D_CPU_EMU)
+  // the code objects are built for gfx950 only (gcnArchName "gfx950:sramecc+:xnack-")
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0 ||
+      (prop.gcnArchName[6] != '\0' && prop.gcnArchName[6] != ':')) {
+    set_err(err, errlen, std::string("FFT3DCL: HIP device ") + std::to_string(device) + " is " +
+                             prop.gcnArchName + "; this library is built for gfx950 (MI355X)");
+    return FFT3D_EDEVICE;
+  }
+#endif
   return FFT3D_OK;
 }
 
@@ -724,7 +748,8 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   std::copy(pl->t.sharpen.begin(), pl->t.sharpen.end(), host.begin() + 3 * nwin + nbin);
   std::copy(pl->t.degrid.begin(), pl->t.degrid.end(), host.begin() + 3 * nwin + 2 * nbin);
 
-  bool ok = hipStreamCreateWithFlags(&pl->own, hipStreamNonBlocking) == hipSuccess;
+  bool ok = hipStreamCreateWithFlags(&pl->own, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&pl->lastCall, hipEventDisableTiming) == hipSuccess;
   ok = ok && pl->alloc(reinterpret_cast<void**>(&pl->dTables), host.size() * sizeof(float));
   ok = ok && pl->alloc(&pl->dStage, (size_t)W * H * pl->bps);
   ok = ok && pl->alloc(reinterpret_cast<void**>(&pl->dBlocks), sizeof(float) * (size_t)pl->nb * bw * bh);
@@ -801,8 +826,13 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
 
 void fft3d_plane_destroy(fft3d_plane* p) {
   if (!p) return;
-  // work queued on caller streams may still read the plane's buffers
-  (void)hipDeviceSynchronize();
+  // Work queued on caller streams may still use the plane's buffers. Every
+  // call ends with lastCall, and a call on a new stream first waits for the
+  // previous one, so lastCall covers all of them; the upload stream and the
+  // plane's own stream are drained too. Other streams' work is not waited for.
+  if (p->anyCall) (void)hipEventSynchronize(p->lastCall);
+  if (p->up) (void)hipStreamSynchronize(p->up);
+  if (p->own) (void)hipStreamSynchronize(p->own);
   (void)hipGetLastError();
   --g_live;
   delete p;
@@ -818,6 +848,7 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
   const size_t need = (size_t)p->d.width * p->d.height * p->bps;
   if (bytes < need) return p->fail(FFT3D_EINVAL, "plane buffer too small");
   hipStream_t s = p->stream(stream);
+  if (!p->order_call(s)) return p->fail(FFT3D_ELAUNCH, "stream wait for the previous call failed");
   fk::c32* spec = p->spectrum(fn, s);
   if (!spec) return p->fail(FFT3D_ENOMEM, "GPU spectra alloc failed");
   const void* plane = src;
@@ -867,6 +898,10 @@ int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst, 
     return rc;
   }
   hipStream_t s = p->stream(stream);
+  if (!p->order_call(s)) {
+    p->pendFn = -1;
+    return p->fail(FFT3D_ELAUNCH, "stream wait for the previous call failed");
+  }
   p->begin_call(s);
   if (!p->wait_prefetched(window, nwin, s)) {
     p->pendFn = -1;
@@ -905,6 +940,7 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
     if (window[i] != fn && !p->ring.count(window[i]))
       return bail(p->fail(FFT3D_EMISSING, "missing spectra in window"));
   hipStream_t s = p->stream(stream);
+  if (!p->order_call(s)) return bail(p->fail(FFT3D_ELAUNCH, "stream wait for the previous call failed"));
   p->begin_call(s);
   const void* plane = src;
   if (!src_on_device) {

@@ -12,7 +12,10 @@
  *     instance mutex around every call, src/fft3dcl.cpp:366);
  *   - `stream` is a hipStream_t (NULL = the plane's own stream). All device
  *     work of a call is ordered on that stream; calls with host buffers
- *     return after the host side is complete.
+ *     return after the host side is complete. A plane's calls share its
+ *     buffers: a call on another stream than the plane's previous call first
+ *     makes its stream wait for that call (an event), so callers may switch
+ *     streams; fft3d_plane_destroy waits for the plane's last call only.
  */
 #ifndef FFT3D_HIP_H
 #define FFT3D_HIP_H
@@ -78,7 +81,7 @@ void fft3d_params_default(fft3d_params* p);
 
 /* Selects and probes HIP device `device` (GpuContext::init,
  * src/cl/cl.cpp:138-181). Returns FFT3D_EDEVICE with a reason when there is
- * no usable gfx950 device. */
+ * no such device or it is not a gfx950 (its gcnArchName is checked). */
 int fft3d_dev_init(int device, char* err, size_t errlen);
 
 /* Validates parameters for this plane (validateParams, src/params.h:76-148)
@@ -88,7 +91,9 @@ int fft3d_dev_init(int device, char* err, size_t errlen);
 int fft3d_plane_create(const fft3d_params* params, const fft3d_plane_desc* desc,
                        fft3d_plane** out, char* err, size_t errlen);
 
-/* Releases everything the plane holds (GpuRef::~GpuRef, src/cl/gpu.cpp:247-270). */
+/* Releases everything the plane holds (GpuRef::~GpuRef, src/cl/gpu.cpp:247-270)
+ * after the plane's last call and its upload stream are done; work of other
+ * planes is not waited for. */
 void fft3d_plane_destroy(fft3d_plane* p);
 
 /* Last error of the plane (GpuRef::error, src/cl/gpu.h:23). */

@@ -545,3 +545,39 @@ def test_emu_fp16_ring_option_errors(emu_lib):
     with pytest.raises(capi.FFT3DError, match="must be 0 or 1"):
         pl.set_option(capi.Plane.OPT_SPECTRA_FP16, 3)
     pl.close()
+
+
+def test_binding_rejects_bad_buffers(emu_lib):
+    """The ctypes binding checks every plane buffer before its address goes
+    to the library: short, strided, wrong element size and host/device
+    mismatches raise ValueError (a short buffer once corrupted the heap)."""
+    W, H = 64, 48
+    clip = make_clip(24, W, H, 3, "u8")[0]
+    pl = capi.Plane(capi.make_params(bt=1), W, H, 8, lib=emu_lib)
+    good = np.ascontiguousarray(clip[0])
+    bad = {
+        "short": np.zeros(64, np.uint8),
+        "strided": np.zeros((H, 2 * W), np.uint8)[:, ::2],
+        "element size": np.zeros((H, W), np.uint16),
+    }
+    for why, buf in bad.items():
+        with pytest.raises(ValueError):
+            pl.forward(0, buf)
+        with pytest.raises(ValueError):
+            pl.forward_process(0, good, [0], buf)
+    with pytest.raises(ValueError):
+        pl.forward(0, good, on_device=True)  # a numpy array is host memory
+    pl.forward(0, good)
+    with pytest.raises(ValueError):
+        pl.process([0], np.zeros(10, np.uint8))
+    out = np.zeros((H, W), np.uint8)
+    pl.process([0], out)
+    assert out.any()
+    # step: a bad output fails before the call, a bad source fails its item
+    with pytest.raises(ValueError):
+        capi.step([capi.StepRequest(pl, 1, 3, lambda fn: good, np.zeros(5, np.uint8))],
+                  lib=emu_lib)
+    st = capi.step([capi.StepRequest(pl, 1, 3, lambda fn: np.zeros(5, np.uint8), out)],
+                   lib=emu_lib)
+    assert st == [-1]
+    pl.close()
