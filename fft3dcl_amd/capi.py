@@ -69,6 +69,7 @@ EXPORTS = [
     "fft3d_plane_prune", "fft3d_plane_clear", "fft3d_live_instances", "fft3d_live_bytes",
     "fft3d_plane_get_info", "fft3d_plane_get_table", "fft3d_plane_read_spectrum",
     "fft3d_plane_forward_process", "fft3d_plane_prefetch", "fft3d_step",
+    "fft3d_plane_process_noalloc",
 ]
 
 
@@ -108,6 +109,7 @@ def load(path: str | Path | None = None) -> C.CDLL:
     L.fft3d_plane_forward_process.argtypes = [vp, i, vp, sz, i, C.POINTER(C.c_int), i, vp, sz,
                                               i, vp]
     L.fft3d_plane_prefetch.argtypes = [vp, i, vp, sz, i]
+    L.fft3d_plane_process_noalloc.argtypes = [vp, C.POINTER(C.c_int), i, vp, sz, vp]
     L.fft3d_step.argtypes = [C.POINTER(WorkItem), i]
     L.fft3d_plane_prune.argtypes = [vp, i, i]
     L.fft3d_plane_prune.restype = None
@@ -248,6 +250,14 @@ class Plane:
                                                      int(on_device), stream))
         finally:
             self._pending = None
+
+    def process_noalloc(self, window, dst, stream: int | None = None):
+        """process() into a device buffer through fft3d_plane_process_noalloc:
+        no device allocation; the kernels' scratch is the library's static
+        array, shared by every such call on the GPU (overlapping calls race)."""
+        w = (C.c_int * len(window))(*window)
+        ptr, n = self.buffer(dst, True, "no-alloc process output")
+        self._check(self.lib.fft3d_plane_process_noalloc(self.h, w, len(window), ptr, n, stream))
 
     def forward_process(self, fn: int, src, window, dst, src_on_device: bool = False,
                         dst_on_device: bool = False, stream: int | None = None):

@@ -20,10 +20,35 @@
 #include <string>
 #include <vector>
 
+// Scratch of fft3d_plane_process_noalloc: the sweep's four partial planes,
+// or the windowed blocks off the sweep path. One copy per GPU, held until the
+// process ends. Host code never passes `scratch` itself to a kernel (in host
+// code the name is a host-side stand-in): it uses the device address that
+// hipGetSymbolAddress returns for the current device, kept in g_scratch.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) float scratch[kScratchBytes / sizeof(float)];
+
 namespace {
 
 std::atomic<int> g_live{0};
 std::atomic<size_t> g_bytes{0};
+
+constexpr int kMaxDevices = 64;
+std::atomic<void*> g_scratch[kMaxDevices];  // device address of `scratch` per device
+
+// Looks up (once per device) the address of `scratch` on device `device`,
+// which must be current. Returns null on failure.
+void* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  void* ptr = g_scratch[device].load();
+  if (ptr) return ptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  g_scratch[device].store(ptr);
+  return ptr;
+}
 
 using fk::AnalysisFn;
 using fk::CombineFn;
@@ -119,6 +144,8 @@ struct fft3d_plane {
   fk::PlaneArgs args{};
   hipStream_t own = nullptr;
   std::string err;
+  const char* errStatic = nullptr;  // set instead of err by fail_static
+  int dev = 0;                      // the device current at create
   size_t bytes = 0;
 
   float* dTables = nullptr;  // windows, pattern, sharpen, degrid
@@ -348,9 +375,9 @@ struct fft3d_plane {
   }
 
   // Chooses the band height and rebuilds the border-tile list for it.
-  bool layoutSweep() {
-    const int nbw = ks.nbw;
-    const int nstrips = (ncols + nbw - 1) / nbw;
+  // Block rows per sweep band for the current options.
+  int bandRowsFor() const {
+    const int nstrips = (ncols + ks.nbw - 1) / ks.nbw;
     int R = bandRows;
     if (R <= 0) {
       // one round of workgroups: as many as the device's CUs hold at once,
@@ -360,8 +387,13 @@ struct fft3d_plane {
       R = (nrows + nbands - 1) / nbands;
       if (R < 2) R = 2;
     }
-    if (R > nrows) R = nrows;
-    if (R == seamR && dSeams) return true;
+    return R > nrows ? nrows : R;
+  }
+  bool layoutSweep() {
+    const int nbw = ks.nbw;
+    const int nstrips = (ncols + nbw - 1) / nbw;
+    const int R = bandRowsFor();
+    if (R == seamR && (dSeams || nseams == 0)) return true;
     const int nbands = (nrows + R - 1) / R;
     const int sx = p.bw / 2, sy = p.bh / 2;
     const int ntx = (d.width + sx - 1) / sx, nty = (d.height + sy - 1) / sy;
@@ -428,25 +460,46 @@ struct fft3d_plane {
 
   int fail(int code, const std::string& why) {
     err = why;
+    errStatic = nullptr;
     return code;
   }
 
   // Window spectra of a process call (errors as fft3d_plane_process).
-  int window(const int* win, int nwin, size_t dstBytes, fk::WindowArgs& w) {
+  // A failure whose message is a string literal: sets no std::string, so
+  // the no-alloc entry point allocates nothing on the host either.
+  int fail_static(int code, const char* why) {
+    err.clear();
+    errStatic = why;
+    return code;
+  }
+  const char* error() const { return errStatic ? errStatic : err.c_str(); }
+
+  // Window spectra of a process call (errors as fft3d_plane_process).
+  int window_args(const int* win, int nwin, size_t dstBytes, fk::WindowArgs& w) {
     const int want = p.bm >= 2 ? p.bt : 1;
-    if (nwin != want) return fail(FFT3D_EINVAL, "window length does not match bt");
+    if (nwin != want) return fail_static(FFT3D_EINVAL, "window length does not match bt");
     const size_t need = (size_t)d.width * d.height * bps;
-    if (dstBytes < need) return fail(FFT3D_EINVAL, "output buffer too small");
+    if (dstBytes < need) return fail_static(FFT3D_EINVAL, "output buffer too small");
     w = fk::WindowArgs{};
     w.nwin = nwin;
     w.cur = p.bm >= 2 ? p.current : 0;
     for (int i = 0; i < 4; i++) {
       auto it = ring.find(win[i < nwin ? i : 0]);
-      if (it == ring.end()) return fail(FFT3D_EMISSING, "missing spectra in window");
+      if (it == ring.end()) return fail_static(FFT3D_EMISSING, "missing spectra in window");
       w.s[i] = it->second;
     }
     return FFT3D_OK;
   }
+  int window(const int* win, int nwin, size_t dstBytes, fk::WindowArgs& w) {
+    return window_args(win, nwin, dstBytes, w);
+  }
+
+  // The kernels of a process call: the sweep and seam combine with partial
+  // planes `part`, or filter/inverse and synthesis through `blocks`. Returns
+  // null, or the launch that failed (the HIP error is in lastLaunchErr).
+  const char* launch(const fk::WindowArgs& w, void* outp, hipStream_t s, const void* fwdPlane,
+                     fk::c32* fwdSpec, int fwdMask, float* part, float* blocks, bool profOk);
+  hipError_t lastLaunchErr = hipSuccess;
 
   // Filter + synthesis of window w into dst. With fwdMask != 0 (sweep only)
   // the sweep also transforms fwdPlane into fwdSpec, the window slots in the
@@ -483,56 +536,74 @@ struct fft3d_plane {
   bool launched(const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
-      err = std::string(what) + " launch failed: " + hipGetErrorString(e);
+      fail(FFT3D_ELAUNCH, std::string(what) + " launch failed: " + hipGetErrorString(e));
       return false;
     }
     return true;
   }
 };
 
-int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipStream_t s,
-                     const void* fwdPlane, fk::c32* fwdSpec, int fwdMask) {
-  const size_t need = (size_t)d.width * d.height * bps;
+const char* fft3d_plane::launch(const fk::WindowArgs& w, void* outp, hipStream_t s,
+                                const void* fwdPlane, fk::c32* fwdSpec, int fwdMask, float* part,
+                                float* blocks, bool profOk) {
   const int kfirst = (p.bm == 0 && kalmanFirst) ? 1 : 0;
-  void* outp = dst_on_device ? dst : dStage;
+  const bool pr = prof && profOk;
+  auto begin = [&]() { return pr ? prof_begin(s) : nullptr; };
+  auto ok = [&]() {
+    lastLaunchErr = hipGetLastError();
+    return lastLaunchErr == hipSuccess;
+  };
   if (sweepOk && sweepOn) {
-    if (!layoutSweep()) return fail(FFT3D_ENOMEM, "GPU alloc failed (seam list)");
     fk::SweepArgs a = sw;
+    a.part = part;
     a.out = outp;
     a.fwdPlane = fwdPlane;
     a.fwdSpec = fwdSpec;
     a.fwdMask = fwdMask;
     SweepFn ksw = fwdMask ? ks.sweepFwd : (half ? ks.sweepH : ks.sweep);
-    hipEvent_t t0 = prof_begin(s);
+    hipEvent_t t0 = begin();
     FK_LAUNCH(ksw, dim3(a.nstrips, a.nbands), dim3(ks.sweepThreads), s, args, w, dKalman, kfirst,
               a);
     prof_end(t0, fwdMask ? K_FSWEEP : K_SWEEP, s);
-    if (!launched(fwdMask ? "analysis/filter/synthesis sweep" : "filter/synthesis sweep"))
-      return FFT3D_ELAUNCH;
+    if (!ok()) return fwdMask ? "analysis/filter/synthesis sweep" : "filter/synthesis sweep";
     if (p.bm == 0) kalmanFirst = false;
     if (a.nseams) {
       CombineFn kc = &fk::k_combine_seams;
-      t0 = prof_begin(s);
+      t0 = begin();
       FK_LAUNCH(kc, dim3((a.nseams + 3) / 4), dim3(256), s, args, a);
       prof_end(t0, K_COMBINE, s);
-      if (!launched("seam combine")) return FFT3D_ELAUNCH;
+      if (!ok()) return "seam combine";
     }
   } else {
-    if (fwdMask) return fail(FFT3D_EINVAL, "fused analysis needs the sweep path");
     const int grid = (nb + ks.G - 1) / ks.G;
     FilterFn kf = half ? ks.filterH : ks.filter_inverse;
-    hipEvent_t t0 = prof_begin(s);
-    FK_LAUNCH(kf, dim3(grid), dim3(64), s, args, w, dKalman, kfirst, dBlocks);
+    hipEvent_t t0 = begin();
+    FK_LAUNCH(kf, dim3(grid), dim3(64), s, args, w, dKalman, kfirst, blocks);
     prof_end(t0, K_FILTER, s);
-    if (!launched("filter/inverse")) return FFT3D_ELAUNCH;
+    if (!ok()) return "filter/inverse";
     if (p.bm == 0) kalmanFirst = false;
     SynthFn ksy = &fk::k_synthesize;
-    t0 = prof_begin(s);
-    FK_LAUNCH(ksy, dim3((d.width + 15) / 16, (d.height + 15) / 16), dim3(256), s, args, dBlocks,
+    t0 = begin();
+    FK_LAUNCH(ksy, dim3((d.width + 15) / 16, (d.height + 15) / 16), dim3(256), s, args, blocks,
               outp);
     prof_end(t0, K_SYNTH, s);
-    if (!launched("synthesize")) return FFT3D_ELAUNCH;
+    if (!ok()) return "synthesize";
   }
+  return nullptr;
+}
+
+int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipStream_t s,
+                     const void* fwdPlane, fk::c32* fwdSpec, int fwdMask) {
+  const size_t need = (size_t)d.width * d.height * bps;
+  void* outp = dst_on_device ? dst : dStage;
+  if (sweepOk && sweepOn) {
+    if (!layoutSweep()) return fail(FFT3D_ENOMEM, "GPU alloc failed (seam list)");
+  } else if (fwdMask) {
+    return fail(FFT3D_EINVAL, "fused analysis needs the sweep path");
+  }
+  if (const char* what = launch(w, outp, s, fwdPlane, fwdSpec, fwdMask, dPart, dBlocks, true))
+    return fail(FFT3D_ELAUNCH,
+                std::string(what) + " launch failed: " + hipGetErrorString(lastLaunchErr));
   if (pendFn >= 0) run_prefetch();
   if (!dst_on_device && download(dst, dStage, need, s) != hipSuccess) {
     (void)hipGetLastError();
@@ -649,6 +720,11 @@ int fft3d_dev_init(int device, char* err, size_t errlen) {
     set_err(err, errlen, "FFT3DCL: hipGetDeviceProperties failed");
     return FFT3D_EDEVICE;
   }
+  // set-up of the no-alloc entry point's scratch on this device
+  if (!scratch_address(device)) {
+    set_err(err, errlen, "FFT3DCL: hipGetSymbolAddress of the scratch array failed");
+    return FFT3D_EDEVICE;
+  }
 #if !defined(FFT3D_CPU_EMU)
   // the code objects are built for gfx950 only (gcnArchName "gfx950:sramecc+:xnack-")
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0 ||
@@ -699,7 +775,8 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   {
     int dev = 0;
     hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    if (hipGetDevice(&dev) == hipSuccess) pl->dev = dev;
+    if (hipGetDeviceProperties(&prop, pl->dev) == hipSuccess)
       pl->ncu = prop.multiProcessorCount;
     (void)hipGetLastError();
   }
@@ -838,7 +915,7 @@ void fft3d_plane_destroy(fft3d_plane* p) {
   delete p;
 }
 
-const char* fft3d_plane_error(const fft3d_plane* p) { return p ? p->err.c_str() : ""; }
+const char* fft3d_plane_error(const fft3d_plane* p) { return p ? p->error() : ""; }
 
 int fft3d_plane_has(const fft3d_plane* p, int fn) { return p && p->ring.count(fn) ? 1 : 0; }
 
@@ -912,6 +989,43 @@ int fft3d_plane_process(fft3d_plane* p, const int* window, int nwin, void* dst, 
   p->pendFn = -1;  // a request the run did not reach is dropped with its buffer
   p->end_call(s);
   return rc;
+}
+
+int fft3d_plane_process_noalloc(fft3d_plane* p, const int* window, int nwin, void* dst,
+                                size_t bytes, void* stream) {
+  if (!p || !window || !dst) return FFT3D_EINVAL;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != p->dev) {
+    (void)hipGetLastError();
+    return p->fail_static(FFT3D_EINVAL, "no-alloc process: the plane's device is not current");
+  }
+  const bool sweep = p->sweepOk && p->sweepOn;
+  const size_t need = sweep ? sizeof(float) * 4 * (size_t)p->sw.ppitch * p->d.height
+                            : sizeof(float) * (size_t)p->nb * p->p.bw * p->p.bh;
+  if (need > kScratchBytes)
+    return p->fail_static(FFT3D_EINVAL,
+                          "no-alloc process: the plane needs more scratch than the 128 MiB "
+                          "array (4 partial planes of W x H floats on the sweep path, "
+                          "nb*bw*bh floats off it)");
+  if (sweep && p->bandRowsFor() != p->seamR)
+    return p->fail_static(FFT3D_EINVAL, "no-alloc process: the sweep's seam list is out of date");
+  float* scr = static_cast<float*>(scratch_address(cur));
+  if (!scr) return p->fail_static(FFT3D_EDEVICE, "no-alloc process: no scratch address");
+  fk::WindowArgs w{};
+  int rc = p->window_args(window, nwin, bytes, w);
+  if (rc != FFT3D_OK) return rc;
+  hipStream_t s = p->stream(stream);
+  if (!p->order_call(s))
+    return p->fail_static(FFT3D_ELAUNCH, "stream wait for the previous call failed");
+  if (!p->wait_prefetched(window, nwin, s)) {
+    (void)hipGetLastError();
+    return p->fail_static(FFT3D_ELAUNCH, "wait for prefetched spectra failed");
+  }
+  // a pending prefetch request stays pending for the next fft3d_plane_process
+  const char* what = p->launch(w, dst, s, nullptr, nullptr, 0, scr, scr, false);
+  p->end_call(s);
+  if (what) return p->fail_static(FFT3D_ELAUNCH, "no-alloc process: a kernel launch failed");
+  return FFT3D_OK;
 }
 
 int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t src_bytes,
@@ -1058,6 +1172,8 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
   case FFT3D_OPT_BAND_ROWS:
     if (value < 0) return p->fail(FFT3D_EINVAL, "band rows must be >= 0");
     p->bandRows = value;
+    // the seam list follows at once, so process calls never rebuild it
+    if (p->sweepOk && !p->layoutSweep()) return p->fail(FFT3D_ENOMEM, "GPU alloc failed (seam list)");
     return FFT3D_OK;
   case FFT3D_OPT_SPECTRA_FP16: {
     if (value != 0 && value != 1)

@@ -129,6 +129,24 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
                                 int src_on_device, const int* window, int nwin, void* dst,
                                 size_t dst_bytes, int dst_on_device, void* stream);
 
+/* fft3d_plane_process with a device output, allocating nothing: no
+ * counterpart in the reference. It is for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The kernels' scratch (the fused sweep's four partial
+ * planes, or the blocks buffer off the sweep path) comes from a static device
+ * array of 128 MiB per GPU, held until the process ends; fft3d_dev_init looks
+ * up its address on each device (else the first call on a device does).
+ * Limit: planes whose scratch exceeds 128 MiB (4 x W x H floats on the sweep
+ * path, nb x bw x bh floats off it; 3840x2160 fits on the sweep path) get
+ * FFT3D_EINVAL. Every call on one GPU shares the array and it keeps the last
+ * call's contents: calls that overlap on one GPU (other streams, other
+ * threads) race, so the caller serialises them. The plane's device must be
+ * current. Same window, errors and output as fft3d_plane_process; no
+ * profiling events, and a pending prefetch request is left for the next
+ * fft3d_plane_process call. */
+int fft3d_plane_process_noalloc(fft3d_plane* p, const int* window, int nwin, void* dst,
+                                size_t bytes, void* stream);
+
 /* forwardRaw(fn, bytes) issued ahead of need (src/cl/gpu.h, forwardRaw; the
  * host step src/fft3dcl.cpp:396-448 forwards each window frame in the call
  * that first needs it). Requests that plane fn go into the ring; the next

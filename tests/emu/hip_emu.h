@@ -189,6 +189,14 @@ inline hipError_t hipGetDevice(int* d) {
   *d = 0;
   return hipSuccess;
 }
+// A file-scope __device__ array is an ordinary host array here; its "device
+// address" is its host address.
+#define HIP_SYMBOL(x) (&(x))
+#define __align__(n) __attribute__((aligned(n)))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+  *p = const_cast<void*>(sym);
+  return hipSuccess;
+}
 inline hipError_t hipGetDeviceCount(int* n) {
   *n = 1;
   return hipSuccess;
