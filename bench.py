@@ -10,15 +10,17 @@ the input planes already resident in HBM and the output written to HBM. In
 steady state one step is one fused launch (the new frame's analysis inside
 the filter/synthesis sweep) plus the seam combine.
 
-Prints one JSON line. Per-kernel device times come from HIP events recorded on
-the launch stream around every kernel of every PROFILE_EVERY-th step of the
-timed region (bracketing every kernel of every step adds about 9 % to the
-step time; the sampled events stay inside the timed region); `roofline`
-prices the dominant kernel's algorithmic bytes per launch (kernel_bytes:
-per-unit figures times the units of one launch) against the 8 TB/s HBM peak. `cpu_baseline` times the
+Prints one JSON line. `value` and `ms_per_step` come from K timed steps with
+no events inside (HIP events on the launch stream before and after them).
+Per-kernel device times come from a second pass of K steps with HIP events
+around every kernel, on the launch stream; `roofline` prices the dominant
+kernel's algorithmic bytes per launch (kernel_bytes: per-unit figures times
+the units of one launch) against the 8 TB/s HBM peak. `config.forms` runs the
+workload in each precision form side by side, each with its error against
+the reference CPU build and the FP64 oracle. `cpu_baseline` times the
 reference's CPU path (its own sources built into oracle/_ref when present,
-else the oracle restatement) single-threaded on a bounded 3-frame sample of
-the same clip.
+else the oracle restatement) single-threaded, in steady state, on configs
+A-D (value: the 4K workload).
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-host] [--no-served]
                     [--no-variants]
@@ -39,7 +41,6 @@ PARAMS = dict(sigma=3.0, bt=4, bw=32, bh=32, ow=16, oh=16, sharpen=0.4)
 HBM_PEAK_GBS = 8000.0
 SEED = 1002
 NUNIQ = 16  # distinct synthetic frames, cycled through the clip
-PROFILE_EVERY = 8  # steps between event-bracketed steps in the timed region
 
 
 def kernel_bytes(info, bps_in, bps_out):
@@ -62,21 +63,101 @@ def kernel_bytes(info, bps_in, bps_out):
     }
 
 
-def cpu_baseline(frames):
-    """Times the reference CPU path on output frames 0..2 (cold window: four
-    analyses + three filter/synthesis passes), one thread."""
+# Configurations of the CPU baseline (the reference's own CPU backend, or the
+# oracle restatement when that build is absent): (width, height, kind, filter,
+# colour, n1, n2). Each is timed over output frames 0..n1-1 and 0..n2-1 from a
+# cold start; (n2 - n1) / (T2 - T1) is the steady-state rate, in which every
+# new output frame costs one analysis and one filter/synthesis pass.
+CPU_CONFIGS = {
+    "A_640x360_u8_bt3": (640, 360, "u8", dict(sigma=2.0, beta=1.0, bt=3, precision=2),
+                         "gray", 4, 16),
+    "B_1080p10_yuv_plane4_bt4": (1920, 1080, "u10",
+                                 dict(sigma=3.0, bt=4, sharpen=0.4, scutoff=0.3, svr=1.0,
+                                      smin=4.0, smax=20.0, plane=4), "yuv", 3, 7),
+    "C_4k10_bt4": (W, H, "u10", PARAMS, "gray", 3, 6),
+    "D_1440p_f32_kalman_bw64": (2560, 1440, "f32",
+                                dict(bt=0, kratio=2.0, bw=64, bh=64, sigma=2.0, sigma2=3.0,
+                                     sigma3=2.0, sigma4=1.0), "gray", 2, 6),
+}
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline():
+    """Steady-state frames/s of the reference CPU path, single-threaded (the
+    reference holds one instance mutex around every plane of every frame), on
+    configs A-D; `value` is config C's (the metric's workload)."""
     sys.path.insert(0, str(ROOT / "tests"))
-    from oracle_lib import Oracle, Ref, script
+    from oracle_lib import Oracle, Ref, script, kind_info
+    from fft3dcl_amd.synth import make_clip
     ref = Ref.load()
     kind = "reference" if ref is not None else "port"
     lib = ref if ref is not None else Oracle()
-    sample = frames[:5]
-    t0 = time.perf_counter()
-    lib.run_clip(script(**PARAMS), [sample], BITS, 0, order=[0, 1, 2])
-    dt = time.perf_counter() - t0
-    return {"value": round(3 / dt, 4), "unit": "frames/s", "cores": 1, "kind": kind,
-            "sample": "output frames 0-2 of the same 4K 10-bit clip from a cold window "
-                      f"(4 analyses + 3 filter/synthesis), {dt:.1f} s single-threaded"}
+    per, total = {}, 0.0
+    for name, (w, h, k, kw, color, n1, n2) in CPU_CONFIGS.items():
+        bits, st = kind_info(k)
+        planes = make_clip(SEED + len(name), w, h, n2 + 2, k, color=color)
+        cf = 3 if color == "yuv" else 1
+        t = []
+        for n in (n1, n2):
+            t0 = time.perf_counter()
+            lib.run_clip(script(**kw), planes, bits, st, color_family=cf, order=list(range(n)))
+            t.append(time.perf_counter() - t0)
+        total += sum(t)
+        per[name] = {"frames_per_s": round((n2 - n1) / (t[1] - t[0]), 4),
+                     "timed": f"frames 0-{n1 - 1} ({t[0]:.2f} s) and 0-{n2 - 1} ({t[1]:.2f} s)"}
+    c = per["C_4k10_bt4"]
+    return {"value": c["frames_per_s"], "unit": "frames/s", "cores": 1, "kind": kind,
+            "sample": "steady state per output frame (one analysis + one filter/synthesis), "
+                      "from the difference of two cold-start runs of n1 and n2 output frames "
+                      f"per config; {total:.1f} s of single-threaded CPU work in all",
+            **cpu_info(), "configs": per}
+
+
+def gpu_configs(steps=40, warmup=6):
+    """Device-resident frames/s of configs A, B and D (the CPU baseline's other
+    configs) on the GPU: planes in HBM, every plane of each output frame,
+    `steps` timed frames cycled over 8 distinct seeded frames."""
+    import torch
+    from fft3dcl_amd.filter import GRAY, YUV, Instance
+    from fft3dcl_amd.synth import make_clip
+    res = {}
+    for name, (w, h, k, kw, color, _, _) in CPU_CONFIGS.items():
+        if name.startswith("C_"):
+            continue
+        bits = {"u8": 8, "u10": 10, "f32": 32}[k]
+        st = 1 if k == "f32" else 0
+        planes = [torch.from_numpy(p).cuda() for p in make_clip(SEED + len(name), w, h, 8, k,
+                                                                  color=color)]
+        cf = YUV if color == "yuv" else GRAY
+        inst = Instance(cf, w, h, bits, st, warmup + steps + 8, **kw)
+        stream = torch.cuda.Stream()
+        outs = [torch.empty_like(p[0]) for p in planes]
+        srcs = [lambda fn, p=p: p[fn % 8] for p in planes]
+        for n in range(warmup):
+            inst.get_frame(n, srcs, outs, on_device=True, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for n in range(warmup, warmup + steps):
+            inst.get_frame(n, srcs, outs, on_device=True, stream=stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        inst.close()
+        ms = e0.elapsed_time(e1) / steps
+        res[name] = {"frames_per_s": round(1000.0 / ms, 1), "ms_per_frame": round(ms, 4)}
+    return res
 
 
 def host_buffer_rates(frames, steps=24):
@@ -119,48 +200,94 @@ def host_buffer_rates(frames, steps=24):
     return res
 
 
-def fp16_ring_variant(dev, kb_fn, steps=100, warmup=10):
-    """The same workload with the spectra ring stored as FP16 (arithmetic
-    FP32): frames/s, per-kernel times (sampled as in the main run), and the
-    largest difference from the FP32 ring's output over 8 frames, in LSB."""
+# Forms of the 4K workload measured side by side: (name, Instance options,
+# plane options). Arithmetic is FP32 in every form; "half" stores the spectra
+# ring as FP16 pairs.
+FORMS = [
+    ("fp32_fft_fp32_ring", dict(), {}),
+    ("fp32_fft_fp16_ring", dict(half=True), {}),
+]
+
+
+def reference_outputs(frames, outs=(1, 2)):
+    """Output frames `outs` of the 4K workload from the reference CPU build
+    (or the oracle when it is absent) and from the FP64 oracle, on the first
+    four frames of the clip (enough for both windows)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_lib import Oracle, Ref, script
+    clip = [frames[:4]]
+    ref = Ref.load()
+    lib = ref if ref is not None else Oracle()
+    r, _ = lib.run_clip(script(**PARAMS), clip, BITS, 0, order=list(outs))
+    o64, _ = Oracle(fp64=True).run_clip(script(**PARAMS), clip, BITS, 0, order=list(outs))
+    return ("reference" if ref is not None else "oracle"), r[0][list(outs)], o64[0][list(outs)]
+
+
+def form_errors(got, ref, o64):
+    import numpy as np
+    g = got.astype(np.int64)
+    dr, d64 = np.abs(g - ref.astype(np.int64)), np.abs(g - o64.astype(np.int64))
+    return {"max_lsb_vs_cpu_reference": int(dr.max()),
+            "identical_pct_vs_cpu_reference": round(100.0 * float((dr == 0).mean()), 3),
+            "max_lsb_vs_fp64_oracle": int(d64.max()),
+            "mean_abs_lsb_vs_fp64_oracle": round(float(d64.mean()), 5)}
+
+
+def forms_section(dev, frames, steps=100, warmup=10, kernel_steps=30):
+    """Each form of FORMS on the 4K workload: frames/s (no events in the timed
+    steps), per-kernel times (a separate pass with events), and the error of
+    output frames 1-2 against the CPU reference and the FP64 oracle, in LSB."""
     import numpy as np
     import torch
     from fft3dcl_amd.filter import GRAY, Instance
-    outs = {}
-    res = {}
-    for half in (False, True):
-        inst = Instance(GRAY, W, H, BITS, 0, steps + warmup + 16, half=half, **PARAMS)
+    kind, ref, o64 = reference_outputs(frames)
+    res = {"errors_on": f"output frames 1-2 of the clip, against the {kind} CPU build "
+                        "and the FP64 oracle"}
+    for name, inst_kw, plane_opts in FORMS:
+        total = steps + warmup + kernel_steps
+        inst = Instance(GRAY, W, H, BITS, 0, total + 16, **inst_kw, **PARAMS)
         plane = inst.objs[0]
+        for opt, val in plane_opts.items():
+            plane.set_option(opt, val)
         stream = torch.cuda.Stream()
         out = torch.empty((H, W), dtype=torch.int16, device="cuda")
+        # error frames: a fresh clip of the same frames 0..3
+        src4 = [lambda fn: dev[min(fn, 3)]]
+        got = []
+        for n in range(3):
+            inst.get_frame(n, src4, [out], on_device=True, stream=stream.cuda_stream)
+            if n >= 1:
+                with torch.cuda.stream(stream):
+                    got.append(out.clone())
+        inst.close()
+        inst = Instance(GRAY, W, H, BITS, 0, total + 16, **inst_kw, **PARAMS)
+        plane = inst.objs[0]
+        for opt, val in plane_opts.items():
+            plane.set_option(opt, val)
         srcs = [lambda fn: dev[fn % NUNIQ]]
-        frames = []
-        for n in range(8):
-            inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
-            with torch.cuda.stream(stream):  # ordered after the frame's kernels
-                frames.append(out.clone())
-        for n in range(8, 8 + warmup):
+        for n in range(warmup):
             inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
         torch.cuda.synchronize()
-        plane.profile()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        for i, n in enumerate(range(8 + warmup, 8 + warmup + steps)):
-            plane.set_profiling(i % PROFILE_EVERY == 0)
+        for n in range(warmup, warmup + steps):
             inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
-        plane.set_profiling(False)
         e1.record(stream)
         torch.cuda.synchronize()
+        plane.profile()
+        plane.set_profiling(True)
+        for n in range(warmup + steps, total):
+            inst.get_frame(n, srcs, [out], on_device=True, stream=stream.cuda_stream)
+        plane.set_profiling(False)
         prof = plane.profile()
         info = plane.info
         inst.close()
-        outs[half] = torch.stack(frames).cpu().numpy().view(np.uint16).astype(np.int32)
-        if half:
-            ms = e0.elapsed_time(e1) / steps
-            res = {"frames_per_s": round(1000.0 / ms, 1), "ms_per_step": round(ms, 4),
-                   "kernels_ms": {k: round(t / c, 4) for k, (c, t) in prof.items()},
-                   "ring_bytes_per_spectrum": info.spectrum_bytes}
-    res["max_lsb_vs_fp32_ring"] = int(np.abs(outs[True] - outs[False]).max())
+        ms = e0.elapsed_time(e1) / steps
+        g = torch.stack(got).cpu().numpy().view(np.uint16)
+        res[name] = {"frames_per_s": round(1000.0 / ms, 1), "ms_per_step": round(ms, 4),
+                     "kernels_ms": {k: round(t / c, 4) for k, (c, t) in prof.items()},
+                     "ring_bytes_per_spectrum": info.spectrum_bytes,
+                     **form_errors(g, ref, o64)}
     return res
 
 
@@ -251,7 +378,7 @@ def main():
     ap.add_argument("--no-served", action="store_true",
                     help="skip the 32-stream served-form sample")
     ap.add_argument("--no-variants", action="store_true",
-                    help="skip the FP16-spectra-ring variant of the workload")
+                    help="skip the precision forms and the device-prefetch variant")
     ap.add_argument("--host-only", action="store_true",
                     help="only measure the host-buffer rates (one JSON line of them)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -289,7 +416,7 @@ def main():
         return
     dev = torch.from_numpy(frames).to("cuda")
     out = torch.empty((H, W), dtype=torch.int16, device="cuda")
-    nframes = max(120, args.warmup + args.steps + 8)
+    nframes = max(120, args.warmup + 2 * args.steps + 8)
     inst = Instance(GRAY, W, H, BITS, 0, nframes, **PARAMS)
     plane = inst.objs[0]
     info = plane.info
@@ -303,18 +430,26 @@ def main():
     torch.cuda.synchronize()
     for n in range(args.warmup):
         step(n)
+    # timed region: K steps, no events inside
     torch.cuda.synchronize()
-    plane.profile()  # discard anything recorded before the timed region
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
-    for i, n in enumerate(range(args.warmup, args.warmup + args.steps)):
-        plane.set_profiling(i % PROFILE_EVERY == 0)
+    for n in range(args.warmup, args.warmup + args.steps):
         step(n)
-    plane.set_profiling(False)
     ev1.record(stream)
     torch.cuda.synchronize()
     ms_total = ev0.elapsed_time(ev1)
+    # kernel pass: the same K steps again with HIP events around every kernel
+    # on the launch stream (they add a few microseconds per step, so they stay
+    # out of the timed region above)
+    base = args.warmup + args.steps
+    plane.profile()
+    plane.set_profiling(True)
+    for n in range(base, base + args.steps):
+        step(n)
+    plane.set_profiling(False)
+    torch.cuda.synchronize()
     prof = plane.profile()
     inst.close()
 
@@ -347,21 +482,23 @@ def main():
                    "step_algorithmic_bytes": kb["step"],
                    "step_GBps": round(step_gbs, 1),
                    "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-                   "kernel_events_every": PROFILE_EVERY,
+                   "kernel_events": "a second pass of the same number of steps after the "
+                                    "timed region, events around every kernel",
                    "kernels": kernels},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
     }
     if not args.no_variants:
-        line["config"]["fp16_spectra_ring"] = fp16_ring_variant(dev, kernel_bytes)
+        line["config"]["forms"] = forms_section(dev, frames)
         line["config"]["device_prefetch"] = prefetch_variant(dev)
+        line["config"]["gpu_configs_A_B_D"] = gpu_configs()
     if not args.no_host:
         line["config"]["host_buffers_pcie_inclusive"] = host_buffer_rates(frames)
     if not args.no_served:
         line["config"]["served_32_streams"] = served_sample()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(frames)
+        line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 

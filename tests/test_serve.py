@@ -93,3 +93,40 @@ def test_gpu_mix_equals_alone(gpu, batched):
     assert s["p99_ms"] >= s["p50_ms"] > 0
     for sp in specs:
         assert np.array_equal(outs[sp.sid], _alone(sp, clips[sp.sid], gpu, True)), sp
+
+
+def _oracle_stream(args):
+    """Oracle output of one served stream (runs in a worker process: the clip
+    is regenerated there from its seed)."""
+    sid, w, h, bits, params, nframes = args
+    from oracle_lib import Oracle, script
+    clip = make_plane_sequence(300 + sid, w, h, nframes, "u8" if bits == 8 else "u10")
+    out, _ = Oracle().run_clip(script(**params), [clip], bits, 0)
+    return sid, out[0]
+
+
+@pytest.mark.gpu
+def test_gpu_config_e_full_shape_vs_oracle(gpu):
+    """The served configuration at its full shape: 32 seeded streams over all
+    four resolutions (640x360 to 3840x2160), 8/10-bit, bt 0/2/3/4 and
+    bw = bh 16/32/64, up to 3 frames each. Every stream's output, per request
+    and batched through fft3d_step, is within the parity bar of the oracle
+    (+/-2 LSB at 8 bit, +/-4 at 10 bit), the two loops agree bit for bit,
+    and each stream alone gives the same frames as in the mix."""
+    import multiprocessing as mp
+    specs = make_specs(1002, 32, duration=0.5, max_frames=3)
+    assert {(s.width, s.height) for s in specs} == {(640, 360), (1280, 720), (1920, 1080),
+                                                    (3840, 2160)}
+    jobs = [(s.sid, s.width, s.height, s.bits, s.params, s.nframes) for s in specs]
+    with mp.get_context("spawn").Pool(8) as pool:
+        want = dict(pool.map(_oracle_stream, jobs))
+    clips = _clips(specs, True)
+    outs, res = _mix(specs, clips, gpu, True, 64 << 30, paced=True)
+    outb, _ = _mix(specs, clips, gpu, True, 64 << 30, paced=False, batched=True)
+    assert res["requests"] == sum(sp.nframes for sp in specs)
+    for sp in specs:
+        tol = 2 if sp.bits == 8 else 4
+        d = int(np.abs(outs[sp.sid].astype(np.int64) - want[sp.sid].astype(np.int64)).max())
+        assert d <= tol, (sp.sid, sp.width, sp.bits, sp.params, d)
+        assert np.array_equal(outb[sp.sid], outs[sp.sid]), sp.sid
+        assert np.array_equal(_alone(sp, clips[sp.sid], gpu, True), outs[sp.sid]), sp.sid
