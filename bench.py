@@ -201,11 +201,16 @@ def host_buffer_rates(frames, steps=24):
 
 
 # Forms of the 4K workload measured side by side: (name, Instance options,
-# plane options). Arithmetic is FP32 in every form; "half" stores the spectra
-# ring as FP16 pairs.
+# plane options). "fp32_fft": the block DFTs as FP32 register FFTs; "mfma":
+# as matrix products on the matrix cores with FP16 or BF16 operands split
+# hi + lo and FP32 accumulation; the filter is FP32 in every form. "half"
+# stores the spectra ring as FP16 pairs.
 FORMS = [
     ("fp32_fft_fp32_ring", dict(), {}),
     ("fp32_fft_fp16_ring", dict(half=True), {}),
+    ("mfma_fp16_split_fp32_ring", dict(dft=1), {}),
+    ("mfma_bf16_split_fp32_ring", dict(dft=2), {}),
+    ("mfma_fp16_split_fp16_ring", dict(dft=1, half=True), {}),
 ]
 
 

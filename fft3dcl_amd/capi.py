@@ -45,7 +45,7 @@ class PlaneInfo(C.Structure):
                 ("bt", C.c_int), ("current", C.c_int), ("ncols", C.c_int),
                 ("nrows", C.c_int), ("blocks_per_wg", C.c_int),
                 ("spectrum_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
-                ("cached_frames", C.c_int)]
+                ("cached_frames", C.c_int), ("dft", C.c_int), ("spectra_fp16", C.c_int)]
 
 
 class KernelTime(C.Structure):
@@ -301,6 +301,7 @@ class Plane:
     OPT_SWEEP, OPT_BAND_ROWS, OPT_FUSED_ANALYSIS, OPT_RESERVE_SPECTRA = 1, 2, 3, 4
     OPT_SPECTRA_FP16 = 5
     OPT_HOST_STAGING, OPT_STAGING_CHUNK = 6, 7
+    OPT_DFT = 8  # 0 register FFTs, 1 MFMA with FP16 hi+lo operands, 2 with BF16
 
     def set_option(self, option: int, value: int):
         self._check(self.lib.fft3d_plane_set_option(self.h, option, int(value)))

@@ -180,6 +180,8 @@ struct fft3d_plane {
   int bandRows = 0;          // option, 0 = automatic
   bool fuseOn = false;       // option: forward_process analyses inside the sweep
   bool half = false;         // option: spectra stored as FP16 pairs
+  int dft = 0;               // option: 0 register FFTs, 1/2 MFMA DFT with FP16/BF16 operands
+  void* dDft = nullptr;      // the MFMA form's constant operands (fk_mfma.h)
   float* dPart = nullptr;    // 4 partial planes
   fk::SeamTile* dSeams = nullptr;  // border tiles of the current band layout
   int nseams = 0, seamR = 0;
@@ -248,6 +250,7 @@ struct fft3d_plane {
     if (up) (void)hipStreamDestroy(up);
     (void)hipFree(dBlocks);
     (void)hipFree(dKalman);
+    (void)hipFree(dDft);
     (void)hipFree(dPart);
     (void)hipFree(dSeams);
     if (own) (void)hipStreamDestroy(own);
@@ -533,6 +536,21 @@ struct fft3d_plane {
     else (void)hipGetLastError();
   }
 
+  // Analysis of `plane` into `spec` on stream s (register FFTs, or the MFMA
+  // form: 32 blocks per 256-thread workgroup).
+  void launch_analysis(const void* plane, fk::c32* spec, hipStream_t s) {
+    hipEvent_t t0 = prof_begin(s);
+    if (dft) {
+      AnalysisFn k = ks.analysisM[dft - 1][half ? 1 : 0];
+      FK_LAUNCH(k, dim3((nb + 31) / 32), dim3(256), s, args, plane, spec);
+    } else {
+      const int grid = (nb + ks.G - 1) / ks.G;
+      AnalysisFn k = half ? ks.analysisH : ks.analysis;
+      FK_LAUNCH(k, dim3(grid), dim3(64), s, args, plane, spec);
+    }
+    prof_end(t0, K_ANALYSIS, s);
+  }
+
   bool launched(const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -561,6 +579,7 @@ const char* fft3d_plane::launch(const fk::WindowArgs& w, void* outp, hipStream_t
     a.fwdSpec = fwdSpec;
     a.fwdMask = fwdMask;
     SweepFn ksw = fwdMask ? ks.sweepFwd : (half ? ks.sweepH : ks.sweep);
+    if (dft && !fwdMask) ksw = ks.sweepM[dft - 1][half ? 1 : 0];
     hipEvent_t t0 = begin();
     FK_LAUNCH(ksw, dim3(a.nstrips, a.nbands), dim3(ks.sweepThreads), s, args, w, dKalman, kfirst,
               a);
@@ -663,11 +682,7 @@ void fft3d_plane::run_prefetch() {
                          : hipMemcpyAsync(dStageUp, src, need, hipMemcpyHostToDevice, up));
   if (e != hipSuccess) return drop();
   const void* plane = onDevice ? src : dStageUp;
-  const int grid = (nb + ks.G - 1) / ks.G;
-  AnalysisFn k = half ? ks.analysisH : ks.analysis;
-  hipEvent_t t0 = prof_begin(up);
-  FK_LAUNCH(k, dim3(grid), dim3(64), up, args, plane, spec);
-  prof_end(t0, K_ANALYSIS, up);
+  launch_analysis(plane, spec, up);
   if (hipGetLastError() != hipSuccess) return drop();
   hipEvent_t ev = nullptr;
   if (!evSpare.empty()) {
@@ -939,11 +954,7 @@ int fft3d_plane_forward(fft3d_plane* p, int fn, const void* src, size_t bytes, i
     }
     plane = p->dStage;
   }
-  const int grid = (p->nb + p->ks.G - 1) / p->ks.G;
-  AnalysisFn k = p->half ? p->ks.analysisH : p->ks.analysis;
-  hipEvent_t t0 = p->prof_begin(s);
-  FK_LAUNCH(k, dim3(grid), dim3(64), s, p->args, plane, spec);
-  p->prof_end(t0, fft3d_plane::K_ANALYSIS, s);
+  p->launch_analysis(plane, spec, s);
   p->end_call(s);
   if (!p->launched("fft2d forward")) {
     p->ring.erase(fn);
@@ -1040,7 +1051,7 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
     p->pendFn = -1;
     return rc;
   };
-  if (!mask || !(p->sweepOk && p->sweepOn && p->fuseOn) || p->half) {
+  if (!mask || !(p->sweepOk && p->sweepOn && p->fuseOn) || p->half || p->dft) {
     const int rc = fft3d_plane_forward(p, fn, src, src_bytes, src_on_device, stream);
     if (rc != FFT3D_OK) return bail(rc);
     return fft3d_plane_process(p, window, nwin, dst, dst_bytes, dst_on_device, stream);
@@ -1205,6 +1216,29 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
       p->pool.push_back(m);
     }
     return FFT3D_OK;
+  case FFT3D_OPT_DFT: {
+    if (value < 0 || value > 2) return p->fail(FFT3D_EINVAL, "DFT form must be 0, 1 or 2");
+    if (value && (!p->ks.analysisM[value - 1][0] || !p->sweepOk))
+      return p->fail(FFT3D_EUNSUPPORTED,
+                     "the MFMA DFT form needs 32x32 blocks on the fused sweep (mode 1, "
+                     "ow = bw/2, oh = bh/2)");
+    if (value && !p->dDft) {
+      const std::vector<uint16_t> t = fft3d::dft_operands32(1);
+      const size_t n = t.size() * sizeof(uint16_t);
+      if (!p->alloc(&p->dDft, 2 * n)) return p->fail(FFT3D_ENOMEM, "GPU alloc failed (DFT table)");
+      const std::vector<uint16_t> tb = fft3d::dft_operands32(2);
+      if (hipMemcpy(p->dDft, t.data(), n, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(static_cast<char*>(p->dDft) + n, tb.data(), n, hipMemcpyHostToDevice) !=
+              hipSuccess) {
+        (void)hipGetLastError();
+        return p->fail(FFT3D_ELAUNCH, "DFT table upload failed");
+      }
+    }
+    p->dft = value;
+    const size_t slots = (size_t)fk::mm::kSlots * 64;  // uint4 per table
+    p->args.dft = value ? reinterpret_cast<const uint4*>(p->dDft) + (value - 1) * slots : nullptr;
+    return FFT3D_OK;
+  }
   case FFT3D_OPT_FUSED_ANALYSIS:
     if (value != 0 && value != 1)
       return p->fail(FFT3D_EINVAL, "fused analysis option must be 0 or 1");
@@ -1294,6 +1328,8 @@ int fft3d_plane_get_info(const fft3d_plane* p, fft3d_plane_info* i) {
   i->spectrum_bytes = p->specBytes;
   i->device_bytes = p->bytes;
   i->cached_frames = (int)p->ring.size();
+  i->dft = p->dft;
+  i->spectra_fp16 = p->half ? 1 : 0;
   return FFT3D_OK;
 }
 

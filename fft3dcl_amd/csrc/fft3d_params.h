@@ -14,6 +14,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -348,6 +349,113 @@ inline Tables buildTables(const Params& p) {
     }
   }
   return t;
+}
+
+// ----- constant operands of the MFMA DFT form (fk_mfma.h), host side -----
+
+// float -> IEEE binary16 / bfloat16 bits, round to nearest even
+inline uint16_t to_f16_bits(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u, ex = (x >> 23) & 0xffu;
+  uint32_t mant = x & 0x7fffffu;
+  if (ex == 0xffu) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0u));
+  const int e = (int)ex - 127 + 15;
+  if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+  if (e <= 0) {
+    if (e < -10) return (uint16_t)sign;
+    mant |= 0x800000u;
+    const int shift = 14 - e;
+    uint32_t h = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+  }
+  uint32_t h = ((uint32_t)e << 10) | (mant >> 13);
+  const uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+  return (uint16_t)(sign | h);
+}
+inline float from_f16_bits(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  uint32_t x;
+  if (e == 0) {
+    const float v = (float)m * 5.9604644775390625e-8f;  // m * 2^-24
+    std::memcpy(&x, &v, 4);
+    x |= sign;
+  } else if (e == 31) {
+    x = sign | 0x7f800000u | (m << 13);
+  } else {
+    x = sign | ((e - 15u + 127u) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+inline uint16_t to_bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+inline float from_bf16_bits(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// The 24 operand slots x 64 lanes x 8 values of the 32-point DFT matrices for
+// the MFMA form (slot layout and lane layout: fk_mfma.h), each entry split
+// into hi = round(c) and lo = round(c - hi) of operand type prec (1 binary16,
+// 2 bfloat16), from matrices evaluated in double.
+inline std::vector<uint16_t> dft_operands32(int prec) {
+  const double kPi = 3.14159265358979323846;
+  auto cs = [&](long long a) { return std::cos(2.0 * kPi * (double)(a % 32) / 32.0); };
+  auto sn = [&](long long a) { return std::sin(2.0 * kPi * (double)(a % 32) / 32.0); };
+  // M[k][m]: real row DFT (column 2p: cos, 2p+1: -sin, column 1: bin 16)
+  auto M = [&](int k, int m) {
+    if (m == 1) return (k & 1) ? -1.0 : 1.0;
+    const int p = m / 2;
+    return (m & 1) ? -sn((long long)k * p) : cs((long long)k * p);
+  };
+  // N[m][k]: real row inverse (row 0: 1, row 1: (-1)^k, 2p: 2cos, 2p+1: -2sin)
+  auto N = [&](int m, int k) {
+    if (m == 0) return 1.0;
+    if (m == 1) return (k & 1) ? -1.0 : 1.0;
+    const int p = m / 2;
+    return (m & 1) ? -2.0 * sn((long long)p * k) : 2.0 * cs((long long)p * k);
+  };
+  auto rho = [](int c, int h, int j) { return 16 * c + 8 * (j >> 2) + 4 * h + (j & 3); };
+  std::vector<uint16_t> out(24 * 64 * 8);
+  auto put = [&](int slot0, int c, int lane, int j, double v) {
+    const float hv = (float)v;
+    uint16_t hi, lo;
+    if (prec == 1) {
+      hi = to_f16_bits(hv);
+      lo = to_f16_bits((float)(v - (double)from_f16_bits(hi)));
+    } else {
+      hi = to_bf16_bits(hv);
+      lo = to_bf16_bits((float)(v - (double)from_bf16_bits(hi)));
+    }
+    out[((size_t)(slot0 + 2 * c) * 64 + lane) * 8 + j] = hi;
+    out[((size_t)(slot0 + 2 * c + 1) * 64 + lane) * 8 + j] = lo;
+  };
+  for (int c = 0; c < 2; c++)
+    for (int lane = 0; lane < 64; lane++) {
+      const int h = lane / 32, i = lane % 32;
+      for (int j = 0; j < 8; j++) {
+        const int kn = 16 * c + 8 * h + j, kr = rho(c, h, j);
+        put(0, c, lane, j, M(kn, i));                          // B: M[k][m = i]
+        put(4, c, lane, j, cs((long long)i * kr));             // B: C[v = i][rho]
+        put(8, c, lane, j, sn((long long)i * kr));             // B: S[v = i][rho]
+        put(12, c, lane, j, cs((long long)kn * i));            // B: C[v = kn][r = i]
+        put(16, c, lane, j, sn((long long)kn * i));            // B: S[v = kn][r = i]
+        put(20, c, lane, j, N(kr, i));                         // A: N[rho][k = i]
+      }
+    }
+  return out;
 }
 
 } // namespace fft3d

@@ -60,6 +60,8 @@ struct PlaneArgs {
   // {shY, gyRe, gyIm, strength}, column c {shX, gxRe, gxIm, 0}
   const float4* sepRow;  // bh
   const float4* sepCol;  // bw/2+1
+  // constant operands of the MFMA DFT form (fk_mfma.h), null when not used
+  const uint4* dft;
 };
 
 // The temporal window handed to the filter: frames in temporal order.

@@ -23,6 +23,9 @@ struct KernelSet {
   AnalysisFn analysisH = nullptr;
   FilterFn filterH = nullptr;
   SweepFn sweepH = nullptr;
+  // the MFMA DFT form (32x32 blocks): [precision - 1][FP16 ring]
+  AnalysisFn analysisM[2][2] = {};
+  SweepFn sweepM[2][2] = {};
   int G = 0;
   int sweepThreads = 0;  // 64 * waves per sweep workgroup
   int nbw = 0;           // block columns per sweep strip

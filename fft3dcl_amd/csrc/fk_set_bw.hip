@@ -26,6 +26,16 @@ KernelSet make_set() {
     k.filterH = &k_filter_inverse<BW, BH, true>;
     k.sweepH = &k_filter_synth<BW, BH, WPW, false, true>;
   }
+  if constexpr (BW == 32 && BH == 32) {
+    k.analysisM[0][0] = &k_analysis_mfma<mm::F16, false>;
+    k.analysisM[0][1] = &k_analysis_mfma<mm::F16, true>;
+    k.analysisM[1][0] = &k_analysis_mfma<mm::BF16, false>;
+    k.analysisM[1][1] = &k_analysis_mfma<mm::BF16, true>;
+    k.sweepM[0][0] = &k_filter_synth<BW, BH, WPW, false, false, mm::F16>;
+    k.sweepM[0][1] = &k_filter_synth<BW, BH, WPW, false, true, mm::F16>;
+    k.sweepM[1][0] = &k_filter_synth<BW, BH, WPW, false, false, mm::BF16>;
+    k.sweepM[1][1] = &k_filter_synth<BW, BH, WPW, false, true, mm::BF16>;
+  }
   k.G = Shape<BW, BH>::G;
   k.sweepThreads = 64 * WPW;
   k.nbw = SweepShape<BW, BH, WPW>::NBW;

@@ -24,7 +24,7 @@
 //      fixed (band, strip) order and packs them.
 #pragma once
 
-#include "fk_kernels.h"
+#include "fk_mfma.h"
 
 namespace fk {
 
@@ -115,7 +115,9 @@ __device__ __forceinline__ void store_sample4(const PlaneArgs& P, void* out, siz
   }
 }
 
-template <int BW, int BH, int WPW, bool FWD, bool H>
+// DFT: 0 register FFTs for the inverse; mm::F16 / mm::BF16 the MFMA form
+// (32x32 blocks, no fused analysis).
+template <int BW, int BH, int WPW, bool FWD, bool H, int DFT = 0>
 __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
                                                            float4* kstate, int kfirst,
                                                            SweepArgs A) {
@@ -169,29 +171,39 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
     }
     filter_block<BW, BH, true, H>(P, Wn, b, colLive, j, Yb, tab, kstate, kfirst != 0,
                                FWD ? A.fwdMask : 0);
-    float xr[BW], xi[BW];
-    inverse_block<BW, BH>(colLive, j, Yb, xr, xi);
-    const bool rowLane = colLive && j < S::NROW;
-    if (rowLane) {
-      // half overlap: both parities use the same separable synthesis window
-      const float fy0 = synY[2 * j], fy1 = synY[2 * j + 1];
-#pragma unroll
-      for (int k = 0; k < BW; k++) {
-        const float fx = synX[k];
-        xr[k] *= fx * fy0;
-        xi[k] *= fx * fy1;
+    if constexpr (DFT != 0) {
+      static_assert(BW == 32 && BH == 32 && !FWD, "the MFMA inverse is for 32x32 blocks");
+      // the wave's blocks one after the other, each by all 64 lanes
+      for (int gg = 0; gg < S::G; gg++) {
+        if (n0 + w * S::G + gg >= P.ncols) break;  // wave-uniform
+        c32* Yg = reinterpret_cast<c32*>(lds) + (w * S::G + gg) * S::YS;
+        mm::inverse_block_mfma<DFT>(P, Yg, S::PITCH, T::OP, synX, synY, lane);
       }
-    }
-    // windowed block -> its own LDS area as rows of OP floats (the area
-    // belongs to this wave, whose inverse reads of it are done)
-    wave_sync();
-    float* const blk = reinterpret_cast<float*>(Yb);
-    if (rowLane) {
-      float* o = blk + (2 * j) * T::OP;
+    } else {
+      float xr[BW], xi[BW];
+      inverse_block<BW, BH>(colLive, j, Yb, xr, xi);
+      const bool rowLane = colLive && j < S::NROW;
+      if (rowLane) {
+        // half overlap: both parities use the same separable synthesis window
+        const float fy0 = synY[2 * j], fy1 = synY[2 * j + 1];
 #pragma unroll
-      for (int k = 0; k < BW; k++) {
-        o[k] = xr[k];
-        o[T::OP + k] = xi[k];
+        for (int k = 0; k < BW; k++) {
+          const float fx = synX[k];
+          xr[k] *= fx * fy0;
+          xi[k] *= fx * fy1;
+        }
+      }
+      // windowed block -> its own LDS area as rows of OP floats (the area
+      // belongs to this wave, whose inverse reads of it are done)
+      wave_sync();
+      float* const blk = reinterpret_cast<float*>(Yb);
+      if (rowLane) {
+        float* o = blk + (2 * j) * T::OP;
+#pragma unroll
+        for (int k = 0; k < BW; k++) {
+          o[k] = xr[k];
+          o[T::OP + k] = xi[k];
+        }
       }
     }
     __syncthreads();  // every block of the strip is in LDS

@@ -49,10 +49,12 @@ class Instance:
     def __init__(self, color_family: int, width: int, height: int, bits: int,
                  sample_type: int, num_frames: int, ssw: int = 1, ssh: int = 1,
                  lib=None, fuse: bool = False, half: bool = False, prefetch: bool = False,
-                 **kw):
+                 dft: int = 0, **kw):
         """fuse: let the filter pass transform the last missing window frame
         itself (FFT3D_OPT_FUSED_ANALYSIS) instead of a separate analysis.
         half: store the spectra ring as FP16 (FFT3D_OPT_SPECTRA_FP16).
+        dft: 0 register FFTs; 1 / 2 the block DFTs on the matrix cores with
+        FP16 / BF16 operands split hi + lo (FFT3D_OPT_DFT, 32x32 blocks).
         prefetch: with bt >= 2, each request also transforms the frame that
         the next sequential request adds to the window (fft3d_plane_prefetch),
         so its upload and analysis overlap this request's work."""
@@ -80,6 +82,8 @@ class Instance:
             o.set_option(capi.Plane.OPT_FUSED_ANALYSIS, int(fuse))
             if half:
                 o.set_option(capi.Plane.OPT_SPECTRA_FP16, 1)
+            if dft:
+                o.set_option(capi.Plane.OPT_DFT, dft)
         info = self.objs[0].info
         self.bm, self.bt, self.current = info.bm, info.bt, info.current
 

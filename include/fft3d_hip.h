@@ -73,6 +73,8 @@ typedef struct fft3d_plane_info {
   size_t spectrum_bytes; /* one frame's spectrum in the ring */
   size_t device_bytes;   /* all device memory the plane holds */
   int cached_frames;     /* spectra currently in the ring */
+  int dft;               /* DFT form (FFT3D_OPT_DFT) */
+  int spectra_fp16;      /* 1: the ring stores FP16 spectra */
 } fft3d_plane_info;
 
 /* Fills the reference's defaults for a clip with no keys set (degrid 1, ow/oh
@@ -262,6 +264,16 @@ int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max);
  *                        Page-locked and device buffers never use the chunks.
  *   FFT3D_OPT_STAGING_CHUNK  bytes per pinned chunk (0 = 4 MiB; else a
  *                        multiple of 64 from 256 to 256 MiB).
+ *   FFT3D_OPT_DFT        0 (default): the block DFTs as register FFTs in
+ *                        FP32; 1: as dense matrix products on the matrix
+ *                        cores (MFMA) with FP16 operands split hi + lo and
+ *                        FP32 accumulation; 2: the same with BF16 operands.
+ *                        Applies to the analysis and to the fused sweep's
+ *                        inverse; 32x32 blocks on the fused sweep only
+ *                        (FFT3D_EUNSUPPORTED otherwise). Replaces the
+ *                        reference's radix-2 fft_batch stages
+ *                        (src/cl/kernels.h:84-129); not exact (error stated
+ *                        per configuration by the tests and the bench).
  * Returns FFT3D_EINVAL for an unknown option or value. */
 enum {
   FFT3D_OPT_SWEEP = 1,
@@ -270,7 +282,8 @@ enum {
   FFT3D_OPT_RESERVE_SPECTRA = 4,
   FFT3D_OPT_SPECTRA_FP16 = 5,
   FFT3D_OPT_HOST_STAGING = 6,
-  FFT3D_OPT_STAGING_CHUNK = 7
+  FFT3D_OPT_STAGING_CHUNK = 7,
+  FFT3D_OPT_DFT = 8
 };
 int fft3d_plane_set_option(fft3d_plane* p, int option, int value);
 

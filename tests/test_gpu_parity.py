@@ -446,3 +446,60 @@ def test_step_host_buffers_equals_host_step(gpu):
         planes[c].close()
         assert want[c].any()
         assert np.array_equal(got[c], want[c])
+
+
+@pytest.mark.parametrize("dft", [1, 2])
+def test_mfma_spectrum_vs_oracle(gpu, oracle, dft):
+    """The MFMA analysis (FP16 or BF16 operands split hi + lo, FP32
+    accumulation) against the oracle's FP32 spectra: relative error below
+    2e-6 (FP16) and 2e-5 (BF16) of the largest bin."""
+    W, H = 640, 360
+    planes = make_clip(41, W, H, 1, "u10")
+    from oracle_lib import Plane as OPlane
+    kw = dict(bt=3, sigma=3.0)
+    inst = Instance(1, W, H, 10, 0, 1, lib=gpu, dft=dft, **kw)
+    pl = inst.objs[0]
+    assert pl.info.dft == dft
+    pl.forward(0, np.ascontiguousarray(planes[0][0]))
+    got = pl.read_spectrum(0)
+    inst.close()
+    op = OPlane(oracle, script(**kw), W, H, 10)
+    want = np.asarray(op.spectra(op.upload(planes[0][0]))).reshape(got.shape)
+    rel = float(np.abs(got - want).max() / np.abs(want).max())
+    print(f"MFMA dft={dft}: spectrum max error {rel:.2e} of the largest bin")
+    assert rel < (2e-6 if dft == 1 else 2e-5)
+
+
+MFMA_CONFIGS = ["A_640x360_u8_bt3", "B_1080p10_yuv_plane4_bt4", "C_4k10_bt4"]
+
+
+@pytest.mark.parametrize("half", [False, True])
+@pytest.mark.parametrize("dft", [1, 2])
+@pytest.mark.parametrize("name", MFMA_CONFIGS)
+def test_mfma_form_configs_vs_oracle(gpu, oracle, oracle64, name, dft, half):
+    """Configs A-C in the MFMA DFT form (FP16 / BF16 hi+lo operands), with the
+    FP32 or the FP16 spectra ring, within the parity bar of the oracle; the
+    error against the FP64 oracle is printed per configuration."""
+    kind, W, H, n, color, kw, cut = CONFIGS[name]
+    bits, st = kind_info(kind)
+    n = min(n, 4)
+    planes = make_clip(1000 + len(name), W, H, n, kind, color=color, scene_cut=cut)
+    cf = 3 if color == "yuv" else 1
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st, color_family=cf)
+    w64, _ = oracle64.run_clip(script(**kw), planes, bits, st, color_family=cf)
+    got = FFT3DCL(planes, color_family=cf, bits=bits, sample_type=st, lib=gpu, dft=dft,
+                  half=half, **kw)
+    d = max(lsb_diff(g, w) for g, w in zip(got, want))
+    d64 = max(lsb_diff(g, w) for g, w in zip(got, w64))
+    print(f"{name} dft={dft} half={half}: max {d} LSB vs oracle, {d64} LSB vs FP64 oracle")
+    for g, w in zip(got, want):
+        check(kind, g, w)
+
+
+def test_mfma_form_option_errors(gpu):
+    for kw in (dict(bw=16, bh=16), dict(mode=0), dict(bw=32, bh=32, ow=10, oh=10)):
+        pl = capi.Plane(capi.make_params(bt=3, **kw), 128, 96, 8, lib=gpu)
+        with pytest.raises(capi.FFT3DError) as e:
+            pl.set_option(capi.Plane.OPT_DFT, 1)
+        assert e.value.code == -6
+        pl.close()
