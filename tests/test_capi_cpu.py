@@ -581,3 +581,48 @@ def test_binding_rejects_bad_buffers(emu_lib):
                    lib=emu_lib)
     assert st == [-1]
     pl.close()
+
+
+@pytest.mark.parametrize("dft", [1, 2])
+@pytest.mark.parametrize("kind,kw", [
+    ("u10", dict(bt=4, sharpen=0.4, sigma=3.0)),
+    ("u8", dict(bt=3)),
+    ("f32", dict(bt=0, sigma2=3.0, sigma3=2.0, sigma4=1.0)),
+    ("u8", dict(bt=2, plane=0)),
+])
+def test_emu_mfma_form_within_parity(oracle, emu_lib, dft, kind, kw):
+    """The MFMA DFT form (emulated matrix products with the gfx950 lane
+    layout, FP16 / BF16 operands split hi + lo) against the oracle: spectra
+    to 2e-6 / 2e-5 of the largest bin, clips within the parity bar; the FP16
+    ring combines with it."""
+    bits, st = kind_info(kind)
+    W, H = 96, 80
+    planes = make_clip(51, W, H, 4, kind)
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st)
+    for half in (False, True):
+        got = FFT3DCL(planes, bits=bits, sample_type=st, lib=emu_lib, dft=dft, half=half, **kw)
+        if kind == "f32":
+            assert max_abs(got[0], want[0]) < 1e-3
+        else:
+            assert lsb_diff(got[0], want[0]) <= (2 if kind == "u8" else 4)
+    pl = capi.Plane(capi.make_params(**kw), W, H, bits, st, lib=emu_lib)
+    pl.set_option(capi.Plane.OPT_DFT, dft)
+    pl.forward(0, np.ascontiguousarray(planes[0][0]))
+    got = pl.read_spectrum(0)
+    op = OPlane(oracle, script(**kw), W, H, bits, st)
+    ref = np.asarray(op.spectra(op.upload(planes[0][0]))).reshape(got.shape)
+    assert np.abs(got - ref).max() / np.abs(ref).max() < (2e-6 if dft == 1 else 2e-5)
+    pl.close()
+
+
+def test_emu_mfma_form_option_errors(emu_lib):
+    for kw in (dict(bw=16, bh=16), dict(mode=0), dict(bw=64, bh=32)):
+        pl = capi.Plane(capi.make_params(bt=3, **kw), 128, 96, 8, lib=emu_lib)
+        with pytest.raises(capi.FFT3DError) as e:
+            pl.set_option(capi.Plane.OPT_DFT, 1)
+        assert e.value.code == -6
+        pl.close()
+    pl = capi.Plane(capi.make_params(bt=3), 128, 96, 8, lib=emu_lib)
+    with pytest.raises(capi.FFT3DError):
+        pl.set_option(capi.Plane.OPT_DFT, 3)
+    pl.close()
