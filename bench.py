@@ -356,17 +356,32 @@ def served_sample(seconds=0.5, nstreams=32, uniq=3):
     sources = {sid: (lambda fn, t=t: t[fn % uniq]) for sid, t in src.items()}
     budget = 64 << 30
     Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=False)  # warm-up
-    cap = Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=False)
-    capb = Server(specs, sources, lambda sid, n: out[sid], budget, batched=True).run(paced=False)
-    paced = Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=True)
-    pacedb = Server(specs, sources, lambda sid, n: out[sid], budget,
-                    batched=True).run(paced=True)
+    def run(paced, **kw):
+        return Server(specs, sources, lambda sid, n: out[sid], budget, **kw).run(paced=paced)
+
+    run(False)  # warm-up of the per-request loop
+    run(False, batched=True, form="fp16_mfma")  # and of the batched one (graph captures)
+    cap = run(False)
+    capb = run(False, batched=True)
+    capbm = run(False, batched=True, form="fp16_mfma")
+    paced = run(True)
+    pacedb = run(True, batched=True)
+    pacedbm = run(True, batched=True, form="fp16_mfma")
     offered = sum(sp.nframes for sp in specs) / seconds
-    res = {"offered_frames_per_s": round(offered, 1), "at_offered_load": summarize(specs, paced),
+    res = {"offered_frames_per_s": round(offered, 1),
+           "at_offered_load": summarize(specs, paced),
            "at_offered_load_batched": summarize(specs, pacedb),
-           "capacity": summarize(specs, cap), "capacity_batched": summarize(specs, capb),
-           "mix": "seeded: resolutions 640x360..3840x2160, 8/10-bit, bt 0/2/3/4, bw 16/32/64"}
-    for k in ("capacity", "capacity_batched"):
+           "at_offered_load_batched_fp16_mfma": summarize(specs, pacedbm),
+           "capacity": summarize(specs, cap),
+           "capacity_batched": summarize(specs, capb),
+           "capacity_batched_fp16_mfma": summarize(specs, capbm),
+           "mix": "seeded: resolutions 640x360..3840x2160, 8/10-bit, bt 0/2/3/4, bw 16/32/64",
+           "forms": "capacity: one call per request, FP32; batched: one fft3d_step call per "
+                    "pass of the loop (batched launches per kernel class, hipGraph replays), "
+                    "FP32 or FP16 ring + MFMA DFT on 32x32 planes"}
+    res["batched_fp16_mfma_over_plain"] = round(
+        res["capacity_batched_fp16_mfma"]["frames_per_s"] / res["capacity"]["frames_per_s"], 3)
+    for k in ("capacity", "capacity_batched", "capacity_batched_fp16_mfma"):
         res[k].pop("p50_ms")
         res[k].pop("p99_ms")
     return res

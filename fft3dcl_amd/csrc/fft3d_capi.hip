@@ -370,11 +370,26 @@ struct fft3d_plane {
     // keep at most bt+2 spare spectra (enough for sequential access); free the rest
     if (pool.size() < (size_t)(p.bt + 2)) {
       pool.push_back(m);
+    } else if (holdFrees) {
+      held.push_back(m);  // a planned launch may still read it
     } else {
       (void)hipFree(m);
       bytes -= specBytes;
       g_bytes -= specBytes;
     }
+  }
+  // While a batched step plans its rounds, spectra that would be freed are
+  // held until its launches are queued.
+  bool holdFrees = false;
+  std::vector<fk::c32*> held;
+  void release_held() {
+    holdFrees = false;
+    for (auto* m : held) {
+      (void)hipFree(m);
+      bytes -= specBytes;
+      g_bytes -= specBytes;
+    }
+    held.clear();
   }
 
   // Chooses the band height and rebuilds the border-tile list for it.
@@ -700,6 +715,460 @@ void fft3d_plane::run_prefetch() {
   }
   upReady[fn] = ev;
 }
+
+namespace {
+
+// ---- the serving step (fft3d_step) ----
+
+bool step_item_valid(const fft3d_work_item& it) {
+  return it.plane && it.source && it.dst && it.num_frames > 0 && it.frame >= 0 &&
+         it.frame < it.num_frames;
+}
+
+// The clamped temporal window of the item's frame and the window frames not
+// in the ring (src/fft3dcl.cpp:319-331, :424-430).
+void step_window(const fft3d_work_item& it, int* window, int& nwin, int* missing, int& nmiss) {
+  const fft3d_plane* p = it.plane;
+  const int bm = p->p.bm, bt = p->p.bt, cur = p->p.current, nf = it.num_frames;
+  auto clampf = [nf](int f) { return f < 0 ? 0 : (f > nf - 1 ? nf - 1 : f); };
+  nwin = bm >= 2 ? bt : 1;
+  nmiss = 0;
+  for (int i = 0; i < nwin; i++) {
+    const int fn = clampf(bm >= 2 ? it.frame - cur + i : it.frame);
+    window[i] = fn;
+    bool dup = false;
+    for (int j = 0; j < nmiss; j++) dup = dup || missing[j] == fn;
+    if (!dup && !p->ring.count(fn)) missing[nmiss++] = fn;
+  }
+}
+
+// The ring rule after an output frame: bt >= 2 keeps [n - current - 1,
+// n - current + bt] (src/fft3dcl.cpp:462); the one-frame modes and Kalman
+// (whose state is not in the ring) keep nothing.
+void step_prune(fft3d_plane* p, int frame) {
+  if (p->p.bm >= 2) fft3d_plane_prune(p, frame - p->p.current - 1, frame - p->p.current + p->p.bt);
+  else fft3d_plane_clear(p);
+}
+
+// One item through the plane calls: forwards of the missing frames, the last
+// one with the filter pass, then the prune.
+int step_item(fft3d_work_item& it) {
+  fft3d_plane* p = it.plane;
+  if (!step_item_valid(it)) {
+    if (p) p->fail(FFT3D_EINVAL, "bad work item");
+    return FFT3D_EINVAL;
+  }
+  int window[4], missing[4], nwin = 0, nmiss = 0, rc = FFT3D_OK;
+  step_window(it, window, nwin, missing, nmiss);
+  const void* src[4] = {};
+  for (int j = 0; j < nmiss && rc == FFT3D_OK; j++)
+    if (!(src[j] = it.source(it.user, missing[j])))
+      rc = p->fail(FFT3D_EINVAL, "work item source gave no plane");
+  for (int j = 0; j + 1 < nmiss && rc == FFT3D_OK; j++)
+    rc = fft3d_plane_forward(p, missing[j], src[j], it.bytes, it.src_on_device, it.stream);
+  if (rc == FFT3D_OK)
+    rc = nmiss ? fft3d_plane_forward_process(p, missing[nmiss - 1], src[nmiss - 1], it.bytes,
+                                             it.src_on_device, window, nwin, it.dst, it.bytes,
+                                             it.dst_on_device, it.stream)
+               : fft3d_plane_process(p, window, nwin, it.dst, it.bytes, it.dst_on_device,
+                                     it.stream);
+  if (rc == FFT3D_OK) step_prune(p, it.frame);
+  return rc;
+}
+
+// Items that go through the batched launches: device sources and outputs, a
+// plane on the fused sweep of the current device with no pending prefetch.
+bool step_batchable(const fft3d_work_item& it, int dev) {
+  const fft3d_plane* p = it.plane;
+  const int hf = p->half ? 1 : 0;
+  return it.src_on_device && it.dst_on_device && p->dev == dev && p->sweepOk && p->sweepOn &&
+         p->pendFn < 0 && p->ks.analysisB[p->dft][hf] && p->ks.sweepB[p->dft][hf];
+}
+
+// Process-wide state of the batched step: its stream, the descriptor buffer
+// in device memory with rotating pinned staging, ordering events, and the
+// hipGraphs of the launch sequences seen so far (a launch sequence is
+// replayed when the same kernels, grids and descriptor offsets recur; the
+// descriptors themselves are uploaded before every launch).
+struct StepEngine {
+  std::mutex mu;
+  int dev = -1;
+  hipStream_t bs = nullptr;
+  hipEvent_t done = nullptr;
+  fk::BatchItem* dItems = nullptr;
+  size_t capItems = 0;
+  static constexpr int NSTAGE = 4;
+  fk::BatchItem* hItems[NSTAGE] = {};
+  size_t hCap[NSTAGE] = {};
+  hipEvent_t hEv[NSTAGE] = {};
+  bool hUsed[NSTAGE] = {};
+  int nextStage = 0;
+  std::vector<hipEvent_t> evs;
+  std::map<std::vector<uintptr_t>, hipGraphExec_t> graphs;
+  int graphsOn = -1;  // FFT3D_STEP_GRAPHS=0 launches directly
+  long graphHits = 0, graphBuilds = 0;
+  std::vector<const void*> loaded;  // kernels whose code object is loaded
+
+  bool init(int device) {
+    if (dev == device && bs) return true;
+    if (bs) return false;  // one device per process for the batched step
+    if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      bs = nullptr;
+      return false;
+    }
+    for (int i = 0; i < NSTAGE; i++)
+      if (hipEventCreateWithFlags(&hEv[i], hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+      }
+    dev = device;
+    if (graphsOn < 0) {
+      const char* e = std::getenv("FFT3D_STEP_GRAPHS");
+      graphsOn = (e && e[0] == '0') ? 0 : 1;
+#if defined(FFT3D_CPU_EMU)
+      graphsOn = 0;  // the emulation runs launches at once: no capture
+#endif
+    }
+    return true;
+  }
+  // streams of the kernel classes of a step call, fork and join events
+  std::vector<hipStream_t> cls;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  bool class_streams(size_t n) {
+    if (!fork && hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    while (cls.size() < n) {
+      hipStream_t st = nullptr;
+      hipEvent_t ev = nullptr;
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+      }
+      cls.push_back(st);
+      join.push_back(ev);
+    }
+    return true;
+  }
+  // ordering event i (created on first use); false if it cannot be created
+  bool event(size_t i, hipEvent_t& e) {
+    while (evs.size() <= i) {
+      hipEvent_t x = nullptr;
+      if (hipEventCreateWithFlags(&x, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+      }
+      evs.push_back(x);
+    }
+    e = evs[i];
+    return true;
+  }
+  void drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
+  // Device copy of the n descriptors (the previous step's kernels on bs are
+  // done with the buffer by the time the copy runs: same stream).
+  bool upload(const std::vector<fk::BatchItem>& v) {
+    const size_t n = v.size();
+    if (n > capItems) {
+      if (hipStreamSynchronize(bs) != hipSuccess) return false;
+      drop_graphs();  // their kernel arguments point at the old buffer
+      (void)hipFree(dItems);
+      dItems = nullptr;
+      capItems = std::max<size_t>(256, 2 * n);
+      if (hipMalloc(reinterpret_cast<void**>(&dItems), capItems * sizeof(fk::BatchItem)) !=
+          hipSuccess) {
+        capItems = 0;
+        return false;
+      }
+    }
+    const int k = nextStage;
+    nextStage = (nextStage + 1) % NSTAGE;
+    if (hUsed[k] && hipEventSynchronize(hEv[k]) != hipSuccess) return false;
+    if (hCap[k] < n) {
+      if (hItems[k]) (void)hipHostFree(hItems[k]);
+      hItems[k] = nullptr;
+      hCap[k] = std::max<size_t>(256, 2 * n);
+      if (hipHostMalloc(reinterpret_cast<void**>(&hItems[k]), hCap[k] * sizeof(fk::BatchItem),
+                        hipHostMallocDefault) != hipSuccess) {
+        hCap[k] = 0;
+        return false;
+      }
+    }
+    std::memcpy(hItems[k], v.data(), n * sizeof(fk::BatchItem));
+    if (hipMemcpyAsync(dItems, hItems[k], n * sizeof(fk::BatchItem), hipMemcpyHostToDevice,
+                       bs) != hipSuccess ||
+        hipEventRecord(hEv[k], bs) != hipSuccess)
+      return false;
+    hUsed[k] = true;
+    return true;
+  }
+};
+StepEngine g_step;
+
+// One batched launch: kernel, workgroup size, descriptors [first, first + n)
+// and total workgroups.
+struct BatchLaunch {
+  fk::BatchFn fn;
+  int threads;
+  size_t first, n;
+  int grid;
+};
+// The launches of one kernel class (block shape, ring precision, DFT form)
+// in one round of a step call: analysis, sweep, seam combine, in that order.
+struct ClassRound {
+  BatchLaunch l[3];
+  std::vector<fk::BatchItem> d[3];
+};
+
+void batch_add(ClassRound& cr, int which, fk::BatchFn fn, int threads, fk::BatchItem bi) {
+  BatchLaunch& l = cr.l[which];
+  if (cr.d[which].empty()) l = BatchLaunch{fn, threads, 0, 0, 0};
+  bi.wg0 = (l.grid + 7) & ~7;  // items start on a multiple of 8 workgroups
+  l.grid = bi.wg0 + bi.nwg;
+  cr.d[which].push_back(bi);
+}
+
+// The batchable items of a step call, round by round (round r: each plane's
+// r-th item). Each kernel class runs its rounds in order on a stream of its
+// own, forked from the step stream and joined back, so classes overlap; the
+// step stream starts after every item's stream and each plane's previous
+// call, and every item's stream continues after the step. With graphs on,
+// the whole launch sequence is one hipGraph (a branch per class), built once
+// per sequence shape and replayed; the descriptors are uploaded before each
+// launch in one copy.
+void step_batch(fft3d_work_item* items, const std::vector<std::vector<int>>& rounds) {
+  StepEngine& E = g_step;
+  std::lock_guard<std::mutex> lk(E.mu);
+  std::vector<int> all_items;
+  for (auto& r : rounds) all_items.insert(all_items.end(), r.begin(), r.end());
+  if (all_items.empty()) return;
+  auto fail_all = [&](int code, const char* why) {
+    for (int k : all_items)
+      if (items[k].status == FFT3D_OK) items[k].status = items[k].plane->fail_static(code, why);
+  };
+  for (int k : all_items) items[k].status = FFT3D_OK;
+  if (!E.init(items[all_items[0]].plane->dev)) {
+    // no step stream: the plane calls serve the items one by one
+    for (auto& r : rounds)
+      for (int k : r) items[k].status = step_item(items[k]);
+    return;
+  }
+  hipStream_t bs = E.bs;
+  std::vector<hipStream_t> streams;
+  for (int k : all_items) {
+    fft3d_plane* p = items[k].plane;
+    hipStream_t s = p->stream(items[k].stream);
+    if (std::find(streams.begin(), streams.end(), s) == streams.end()) streams.push_back(s);
+    if (!p->order_call(bs)) return fail_all(FFT3D_ELAUNCH, "stream wait failed");
+  }
+  for (size_t i = 0; i < streams.size(); i++) {
+    hipEvent_t e = nullptr;
+    if (!E.event(i, e) || hipEventRecord(e, streams[i]) != hipSuccess ||
+        hipStreamWaitEvent(bs, e, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail_all(FFT3D_ELAUNCH, "step stream ordering failed");
+    }
+  }
+  // host side of every round: windows, sources, ring entries, descriptors
+  for (int k : all_items) items[k].plane->holdFrees = true;
+  struct Release {  // frees held spectra once the launches are queued (or failed)
+    fft3d_work_item* items;
+    const std::vector<int>& ks;
+    ~Release() {
+      for (int k : ks) items[k].plane->release_held();
+    }
+  } release{items, all_items};
+  std::vector<fk::BatchFn> classes;            // by sweep kernel
+  std::vector<std::vector<ClassRound>> plan;   // [class][round]
+  std::vector<std::pair<fft3d_plane*, int>> added;  // ring entries the analyses fill
+  for (size_t r = 0; r < rounds.size(); r++) {
+    for (int k : rounds[r]) {
+      fft3d_work_item& it = items[k];
+      fft3d_plane* p = it.plane;
+      int window[4], missing[4], nwin = 0, nmiss = 0;
+      step_window(it, window, nwin, missing, nmiss);
+      const void* src[4] = {};
+      bool ok = true;
+      for (int j = 0; j < nmiss && ok; j++)
+        if (!(src[j] = it.source(it.user, missing[j]))) {
+          it.status = p->fail_static(FFT3D_EINVAL, "work item source gave no plane");
+          ok = false;
+        }
+      const size_t need = (size_t)p->d.width * p->d.height * p->bps;
+      if (ok && it.bytes < need) {
+        it.status = p->fail_static(FFT3D_EINVAL, "plane buffer too small");
+        ok = false;
+      }
+      if (ok && !p->layoutSweep()) {
+        it.status = p->fail_static(FFT3D_ENOMEM, "GPU alloc failed (seam list)");
+        ok = false;
+      }
+      fk::c32* spec[4] = {};
+      for (int j = 0; j < nmiss && ok; j++)
+        if (!(spec[j] = p->spectrum(missing[j], bs))) {
+          it.status = p->fail_static(FFT3D_ENOMEM, "GPU spectra alloc failed");
+          ok = false;
+        }
+      fk::WindowArgs w{};
+      if (ok && (it.status = p->window_args(window, nwin, it.bytes, w)) != FFT3D_OK) ok = false;
+      if (ok && !p->wait_prefetched(window, nwin, bs)) {
+        (void)hipGetLastError();
+        it.status = p->fail_static(FFT3D_ELAUNCH, "wait for prefetched spectra failed");
+        ok = false;
+      }
+      if (!ok) {
+        for (int j = 0; j < nmiss; j++)
+          if (spec[j]) {
+            p->ring.erase(missing[j]);
+            p->retire(spec[j]);
+          }
+        continue;
+      }
+      const int hf = p->half ? 1 : 0;
+      const fk::BatchFn sfn = p->ks.sweepB[p->dft][hf];
+      size_t c = std::find(classes.begin(), classes.end(), sfn) - classes.begin();
+      if (c == classes.size()) {
+        classes.push_back(sfn);
+        plan.emplace_back();
+      }
+      if (plan[c].size() <= r) plan[c].resize(r + 1);
+      ClassRound& cr = plan[c][r];
+      for (int j = 0; j < nmiss; j++) {
+        added.emplace_back(p, missing[j]);
+        fk::BatchItem bi{};
+        bi.P = p->args;
+        bi.plane = src[j];
+        bi.spec = spec[j];
+        bi.nwg = p->dft ? (p->nb + 31) / 32 : (p->nb + p->ks.G - 1) / p->ks.G;
+        batch_add(cr, 0, p->ks.analysisB[p->dft][hf], p->dft ? 256 : 64, bi);
+      }
+      fk::BatchItem bi{};
+      bi.P = p->args;
+      bi.Wn = w;
+      bi.A = p->sw;
+      bi.A.part = p->dPart;
+      bi.A.out = it.dst;
+      bi.A.fwdPlane = nullptr;
+      bi.A.fwdSpec = nullptr;
+      bi.A.fwdMask = 0;
+      bi.kstate = p->dKalman;
+      bi.kfirst = (p->p.bm == 0 && p->kalmanFirst) ? 1 : 0;
+      bi.nwg = bi.A.nstrips * bi.A.nbands;
+      batch_add(cr, 1, sfn, p->ks.sweepThreads, bi);
+      if (bi.A.nseams) {
+        bi.nwg = (bi.A.nseams + 3) / 4;
+        batch_add(cr, 2, &fk::k_combine_batch, 256, bi);
+      }
+      // the ring and Kalman bookkeeping of this item, before the plane's next round
+      if (p->p.bm == 0) p->kalmanFirst = false;
+      step_prune(p, it.frame);
+    }
+  }
+  // one descriptor array for the whole call, in launch order
+  std::vector<fk::BatchItem> desc;
+  std::vector<std::vector<BatchLaunch>> seq(plan.size());
+  for (size_t c = 0; c < plan.size(); c++)
+    for (ClassRound& cr : plan[c])
+      for (int q = 0; q < 3; q++) {
+        if (cr.d[q].empty()) continue;
+        cr.l[q].first = desc.size();
+        cr.l[q].n = cr.d[q].size();
+        desc.insert(desc.end(), cr.d[q].begin(), cr.d[q].end());
+        seq[c].push_back(cr.l[q]);
+      }
+  bool ok = desc.empty() || E.upload(desc);
+  if (ok && !desc.empty()) {
+    ok = E.class_streams(plan.size());
+    // fork to the class streams, their launches, join back to the step stream
+    auto launch_all = [&]() {
+      if (hipEventRecord(E.fork, bs) != hipSuccess) return false;
+      for (size_t c = 0; c < seq.size(); c++) {
+        hipStream_t cs = E.cls[c];
+        if (hipStreamWaitEvent(cs, E.fork, 0) != hipSuccess) return false;
+        for (const BatchLaunch& l : seq[c])
+          FK_LAUNCH(l.fn, dim3(l.grid), dim3(l.threads), cs, E.dItems + l.first, (int)l.n);
+        if (hipEventRecord(E.join[c], cs) != hipSuccess ||
+            hipStreamWaitEvent(bs, E.join[c], 0) != hipSuccess)
+          return false;
+      }
+      return hipGetLastError() == hipSuccess;
+    };
+    if (!ok) {
+    } else if (!E.graphsOn) {
+      ok = launch_all();
+    } else {
+      // a capture must not be the first use of a kernel (its code object
+      // is loaded then): query each kernel once beforehand
+      std::vector<uintptr_t> key{reinterpret_cast<uintptr_t>(E.dItems)};
+      for (size_t c = 0; c < seq.size(); c++) {
+        key.push_back(~(uintptr_t)c);
+        for (const BatchLaunch& l : seq[c]) {
+          const void* f = reinterpret_cast<const void*>(l.fn);
+          if (std::find(E.loaded.begin(), E.loaded.end(), f) == E.loaded.end()) {
+            hipFuncAttributes attr;
+            if (hipFuncGetAttributes(&attr, f) != hipSuccess) (void)hipGetLastError();
+            else E.loaded.push_back(f);
+          }
+          key.insert(key.end(), {reinterpret_cast<uintptr_t>(l.fn), (uintptr_t)l.grid,
+                                 (uintptr_t)l.threads, (uintptr_t)l.first, (uintptr_t)l.n});
+        }
+      }
+      auto g = E.graphs.find(key);
+      if (g == E.graphs.end()) {
+        if (E.graphs.size() >= 256) E.drop_graphs();
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        ok = hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+          ok = launch_all();
+          ok = hipStreamEndCapture(bs, &graph) == hipSuccess && ok;
+        }
+        ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ok) {
+          g = E.graphs.emplace(key, exec).first;
+          E.graphBuilds++;
+        }
+      } else {
+        E.graphHits++;
+      }
+      ok = ok && hipGraphLaunch(g->second, bs) == hipSuccess;
+    }
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    // the analyses did not run: their ring entries hold nothing
+    for (auto& pa : added) {
+      auto r = pa.first->ring.find(pa.second);
+      if (r != pa.first->ring.end()) {
+        pa.first->retire(r->second);
+        pa.first->ring.erase(r);
+      }
+    }
+    return fail_all(FFT3D_ELAUNCH, "batched step launch failed");
+  }
+  // later work on each item's stream comes after the step
+  if (hipEventRecord(E.done, bs) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail_all(FFT3D_ELAUNCH, "step stream ordering failed");
+  }
+  for (hipStream_t s : streams)
+    if (hipStreamWaitEvent(s, E.done, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail_all(FFT3D_ELAUNCH, "step stream ordering failed");
+    }
+  for (int k : all_items)
+    if (items[k].status == FFT3D_OK) items[k].plane->end_call(bs);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1102,50 +1571,33 @@ int fft3d_plane_forward_process(fft3d_plane* p, int fn, const void* src, size_t 
 
 int fft3d_step(fft3d_work_item* items, int n) {
   if (!items || n < 0) return FFT3D_EINVAL;
-  int first = FFT3D_OK;
+  // Round r holds each plane's r-th item. A plane whose items are all
+  // device-resident, on the fused sweep of the current device, goes through
+  // the batched launches (step_batch); the others run item by item
+  // (step_item), in order. The two groups share no plane.
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) (void)hipGetLastError();
+  std::map<const fft3d_plane*, std::pair<int, bool>> pl;  // items so far, all batchable
+  std::vector<int> round(n, 0);
   for (int k = 0; k < n; k++) {
-    fft3d_work_item& it = items[k];
-    fft3d_plane* p = it.plane;
-    int rc = FFT3D_OK;
-    if (!p || !it.source || !it.dst || it.num_frames <= 0 || it.frame < 0 ||
-        it.frame >= it.num_frames) {
-      rc = FFT3D_EINVAL;
-      if (p) p->fail(rc, "bad work item");
-    } else {
-      const int bm = p->p.bm, bt = p->p.bt, cur = p->p.current, nf = it.num_frames;
-      auto clampf = [nf](int f) { return f < 0 ? 0 : (f > nf - 1 ? nf - 1 : f); };
-      int window[4], missing[4], nwin = bm >= 2 ? bt : 1, nmiss = 0;
-      for (int i = 0; i < nwin; i++) {
-        const int fn = clampf(bm >= 2 ? it.frame - cur + i : it.frame);
-        window[i] = fn;
-        bool seen = false;
-        for (int j = 0; j < nmiss; j++) seen = seen || missing[j] == fn;
-        if (!seen && !p->ring.count(fn)) missing[nmiss++] = fn;
-      }
-      const void* src[4] = {};
-      for (int j = 0; j < nmiss && rc == FFT3D_OK; j++)
-        if (!(src[j] = it.source(it.user, missing[j])))
-          rc = p->fail(FFT3D_EINVAL, "work item source gave no plane");
-      // the missing frames but the last through forward, the last with the
-      // filter pass (forward_process)
-      for (int j = 0; j + 1 < nmiss && rc == FFT3D_OK; j++)
-        rc = fft3d_plane_forward(p, missing[j], src[j], it.bytes, it.src_on_device, it.stream);
-      if (rc == FFT3D_OK)
-        rc = nmiss ? fft3d_plane_forward_process(p, missing[nmiss - 1], src[nmiss - 1], it.bytes,
-                                                 it.src_on_device, window, nwin, it.dst,
-                                                 it.bytes, it.dst_on_device, it.stream)
-                   : fft3d_plane_process(p, window, nwin, it.dst, it.bytes, it.dst_on_device,
-                                         it.stream);
-      if (rc == FFT3D_OK) {
-        // Kalman state lives in the plane's Kalman buffer: like the
-        // one-frame modes, bt=0 reuses no ring entry
-        if (bm >= 2) fft3d_plane_prune(p, it.frame - cur - 1, it.frame - cur + bt);
-        else fft3d_plane_clear(p);
-      }
-    }
-    it.status = rc;
-    if (rc != FFT3D_OK && first == FFT3D_OK) first = rc;
+    const bool b = step_item_valid(items[k]) && step_batchable(items[k], cur);
+    auto& e = pl.emplace(items[k].plane, std::make_pair(0, true)).first->second;
+    round[k] = e.first++;
+    e.second = e.second && b;
   }
+  std::vector<std::vector<int>> rounds;
+  for (int k = 0; k < n; k++) {
+    if (!items[k].plane || !pl[items[k].plane].second) {
+      items[k].status = step_item(items[k]);
+      continue;
+    }
+    if ((int)rounds.size() <= round[k]) rounds.resize(round[k] + 1);
+    rounds[round[k]].push_back(k);
+  }
+  if (!rounds.empty()) step_batch(items, rounds);
+  int first = FFT3D_OK;
+  for (int k = 0; k < n; k++)
+    if (items[k].status != FFT3D_OK && first == FFT3D_OK) first = items[k].status;
   return first;
 }
 

@@ -297,12 +297,14 @@ __device__ __forceinline__ int xcd_contiguous(int i, int n) {
   return x * q + min(x, r) + i / 8;
 }
 
+// Analysis of the G blocks of work item wg of nwg (one 64-lane workgroup).
 template <int BW, int BH, bool H>
-__global__ __launch_bounds__(64) void k_analysis(PlaneArgs P, const void* plane, c32* spec) {
+__device__ __forceinline__ void analysis_body(const PlaneArgs& P, const void* plane, c32* spec,
+                                              int wg, int nwg) {
   using S = Shape<BW, BH>;
   __shared__ c32 lds[S::LDS_A];
   const int lane = threadIdx.x, g = lane / S::LPB, j = lane - g * S::LPB;
-  const int b = xcd_contiguous(blockIdx.x, gridDim.x) * S::G + g;
+  const int b = xcd_contiguous(wg, nwg) * S::G + g;
   const bool live = b < P.nb;
   const size_t d = (size_t)b * S::BSTR + j;
   forward_block<BW, BH, S::SPLIT_A>(P, plane, b, live, g, j, lds,
@@ -311,6 +313,10 @@ __global__ __launch_bounds__(64) void k_analysis(PlaneArgs P, const void* plane,
                                       for (int r = 0; r < BH; r++)
                                         spec_store<H>(spec, d + r * S::NCOL, cmake(yr[r], yi[r]));
                                     });
+}
+template <int BW, int BH, bool H>
+__global__ __launch_bounds__(64) void k_analysis(PlaneArgs P, const void* plane, c32* spec) {
+  analysis_body<BW, BH, H>(P, plane, spec, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- filter

@@ -359,14 +359,19 @@ __device__ __forceinline__ void inverse_block_mfma(const PlaneArgs& P, c32* Yb, 
 // 4 at a time to its waves, so neighbouring blocks, which share pixel rows,
 // are read together.
 template <int PR, bool H>
-__global__ __launch_bounds__(256) void k_analysis_mfma(PlaneArgs P, const void* plane, c32* spec) {
+__device__ __forceinline__ void analysis_mfma_body(const PlaneArgs& P, const void* plane,
+                                                   c32* spec, int wg, int nwg) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int g = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int g = xcd_contiguous(wg, nwg);
   for (int i = 0; i < 8; i++) {
     const int b = g * 32 + i * 4 + wv;
     if (b >= P.nb) break;  // wave-uniform
     mm::forward_block_mfma<PR, H>(P, plane, b, spec, lane);
   }
+}
+template <int PR, bool H>
+__global__ __launch_bounds__(256) void k_analysis_mfma(PlaneArgs P, const void* plane, c32* spec) {
+  analysis_mfma_body<PR, H>(P, plane, spec, blockIdx.x, gridDim.x);
 }
 
 }  // namespace fk

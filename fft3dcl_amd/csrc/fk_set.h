@@ -4,7 +4,7 @@
 // parallel.
 #pragma once
 
-#include "fk_sweep.h"
+#include "fk_batch.h"
 
 namespace fk {
 
@@ -13,6 +13,7 @@ using FilterFn = void (*)(PlaneArgs, WindowArgs, float4*, int, float*);
 using SynthFn = void (*)(PlaneArgs, const float*, void*);
 using SweepFn = void (*)(PlaneArgs, WindowArgs, float4*, int, SweepArgs);
 using CombineFn = void (*)(PlaneArgs, SweepArgs);
+using BatchFn = void (*)(const BatchItem*, int);
 
 struct KernelSet {
   AnalysisFn analysis = nullptr;
@@ -26,6 +27,9 @@ struct KernelSet {
   // the MFMA DFT form (32x32 blocks): [precision - 1][FP16 ring]
   AnalysisFn analysisM[2][2] = {};
   SweepFn sweepM[2][2] = {};
+  // batched launches (fft3d_step): [DFT form 0..2][FP16 ring]
+  BatchFn analysisB[3][2] = {};
+  BatchFn sweepB[3][2] = {};
   int G = 0;
   int sweepThreads = 0;  // 64 * waves per sweep workgroup
   int nbw = 0;           // block columns per sweep strip

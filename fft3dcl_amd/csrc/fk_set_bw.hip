@@ -21,10 +21,14 @@ KernelSet make_set() {
   k.filter_inverse = &k_filter_inverse<BW, BH, false>;
   k.sweep = &k_filter_synth<BW, BH, WPW, false, false>;
   k.sweepFwd = &k_filter_synth<BW, BH, WPW, true, false>;
+  k.analysisB[0][0] = &k_analysis_batch<BW, BH, false>;
+  k.sweepB[0][0] = &k_sweep_batch<BW, BH, WPW, false, 0>;
   if constexpr (BW == BH && BW >= 16) {
     k.analysisH = &k_analysis<BW, BH, true>;
     k.filterH = &k_filter_inverse<BW, BH, true>;
     k.sweepH = &k_filter_synth<BW, BH, WPW, false, true>;
+    k.analysisB[0][1] = &k_analysis_batch<BW, BH, true>;
+    k.sweepB[0][1] = &k_sweep_batch<BW, BH, WPW, true, 0>;
   }
   if constexpr (BW == 32 && BH == 32) {
     k.analysisM[0][0] = &k_analysis_mfma<mm::F16, false>;
@@ -35,6 +39,14 @@ KernelSet make_set() {
     k.sweepM[0][1] = &k_filter_synth<BW, BH, WPW, false, true, mm::F16>;
     k.sweepM[1][0] = &k_filter_synth<BW, BH, WPW, false, false, mm::BF16>;
     k.sweepM[1][1] = &k_filter_synth<BW, BH, WPW, false, true, mm::BF16>;
+    k.analysisB[1][0] = &k_analysis_mfma_batch<mm::F16, false>;
+    k.analysisB[1][1] = &k_analysis_mfma_batch<mm::F16, true>;
+    k.analysisB[2][0] = &k_analysis_mfma_batch<mm::BF16, false>;
+    k.analysisB[2][1] = &k_analysis_mfma_batch<mm::BF16, true>;
+    k.sweepB[1][0] = &k_sweep_batch<BW, BH, WPW, false, mm::F16>;
+    k.sweepB[1][1] = &k_sweep_batch<BW, BH, WPW, true, mm::F16>;
+    k.sweepB[2][0] = &k_sweep_batch<BW, BH, WPW, false, mm::BF16>;
+    k.sweepB[2][1] = &k_sweep_batch<BW, BH, WPW, true, mm::BF16>;
   }
   k.G = Shape<BW, BH>::G;
   k.sweepThreads = 64 * WPW;

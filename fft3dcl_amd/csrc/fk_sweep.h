@@ -115,19 +115,19 @@ __device__ __forceinline__ void store_sample4(const PlaneArgs& P, void* out, siz
   }
 }
 
+// The sweep of one workgroup: strip `strip`, band `band` of the plane.
 // DFT: 0 register FFTs for the inverse; mm::F16 / mm::BF16 the MFMA form
 // (32x32 blocks, no fused analysis).
-template <int BW, int BH, int WPW, bool FWD, bool H, int DFT = 0>
-__global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
-                                                           float4* kstate, int kfirst,
-                                                           SweepArgs A) {
+template <int BW, int BH, int WPW, bool FWD, bool H, int DFT>
+__device__ __forceinline__ void sweep_body(const PlaneArgs& P, const WindowArgs& Wn,
+                                           float4* kstate, int kfirst, const SweepArgs& A,
+                                           int strip, int band) {
   using S = Shape<BW, BH>;
   using T = SweepShape<BW, BH, WPW>;
   constexpr int SX = T::SX, SY = T::SY, NBW = T::NBW, CW = T::CW;
   __shared__ __attribute__((aligned(16))) float lds[T::LDS_FLOATS];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int g = lane / S::LPB, j = lane - g * S::LPB;
-  const int strip = blockIdx.x, band = blockIdx.y;
   const int n0 = strip * NBW, m0 = band * A.R;
   const int mEnd = min(m0 + A.R, P.nrows);
   const int nlive = min(NBW, P.ncols - n0);
@@ -349,14 +349,19 @@ __global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, Windo
   }
 }
 
-#ifdef FK_COMMON_KERNELS
-// Adds the partial sums of border tiles from every workgroup that touched
-// them, in (band, strip) order (the planes listed with each tile), adds
-// chroma centring and packs. One wave per tile, four samples per lane when
-// the tile width is a multiple of 4.
-__global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A) {
+template <int BW, int BH, int WPW, bool FWD, bool H, int DFT = 0>
+__global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
+                                                           float4* kstate, int kfirst,
+                                                           SweepArgs A) {
+  sweep_body<BW, BH, WPW, FWD, H, DFT>(P, Wn, kstate, kfirst, A, blockIdx.x, blockIdx.y);
+}
+
+// Adds the partial sums of border tile si from every workgroup that touched
+// it, in (band, strip) order (the planes listed with the tile), adds chroma
+// centring and packs: one wave per tile, four samples per lane when the tile
+// width is a multiple of 4.
+__device__ __forceinline__ void combine_body(const PlaneArgs& P, const SweepArgs& A, int si) {
   const int lane = threadIdx.x & 63;
-  const int si = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (si >= A.nseams) return;  // no barriers below
   const SeamTile tile = A.seams[si];
   const int sx = P.bw / 2, sy = P.bh / 2;
@@ -397,6 +402,11 @@ __global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A)
       store_sample(P, A.out, (size_t)y * P.W + x, P.chromaNorm != 0.0f ? v + P.chromaNorm : v);
     }
   }
+}
+
+#ifdef FK_COMMON_KERNELS
+__global__ __launch_bounds__(256) void k_combine_seams(PlaneArgs P, SweepArgs A) {
+  combine_body(P, A, blockIdx.x * 4 + (threadIdx.x >> 6));
 }
 #endif  // FK_COMMON_KERNELS
 

@@ -98,6 +98,21 @@ def footprint(spec: StreamSpec) -> int:
     return spec_bytes * slots + kalman + planes + blocks
 
 
+def form_options(form: str, spec: StreamSpec) -> dict:
+    """Instance options of a served form for one stream."""
+    if form == "fp32":
+        return {}
+    if form != "fp16_mfma":
+        raise ValueError(f"unknown served form {form!r}")
+    bw, bh = spec.params["bw"], spec.params["bh"]
+    opts = {}
+    if bw == bh and bw in (16, 32, 64):
+        opts["half"] = True
+    if bw == bh == 32:
+        opts["dft"] = 1
+    return opts
+
+
 class Server:
     """Runs the requests of `specs` with at most `budget` estimated device
     bytes of admitted streams. `sources[sid](fn)` gives frame fn of stream
@@ -105,8 +120,12 @@ class Server:
     (sid, n) -> buffer receives the filtered frame."""
 
     def __init__(self, specs, sources, outputs, budget: int, lib=None, on_device=True,
-                 batched: bool = False):
+                 batched: bool = False, form: str = "fp32"):
+        """form: "fp32" (FP32 register FFTs and spectra ring) or "fp16_mfma"
+        (FP16 spectra ring on every square 16/32/64 plane, and the MFMA DFT
+        with FP16 hi+lo operands on the 32x32 ones)."""
         self.batched = batched
+        self.form = form
         self.specs = {s.sid: s for s in specs}
         self.sources = sources
         self.outputs = outputs
@@ -139,7 +158,7 @@ class Server:
                 waiting.pop(0)
                 sp = self.specs[sid]
                 inst = Instance(GRAY, sp.width, sp.height, sp.bits, 0, sp.nframes,
-                                lib=self.lib, **sp.params)
+                                lib=self.lib, **form_options(self.form, sp), **sp.params)
                 # the ring's spectra are allocated now, not inside the loop
                 inst.objs[0].set_option(inst.objs[0].OPT_RESERVE_SPECTRA,
                                         max(sp.params["bt"], 1) + 2)

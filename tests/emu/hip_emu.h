@@ -272,5 +272,26 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
   return hipSuccess;
 }
 
+// hipGraph API subset: present so that the library compiles; the emulation
+// runs every launch at once, so the library never captures under it.
+typedef void* hipGraph_t;
+typedef void* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1 };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return 1; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return 1; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, void*, size_t) {
+  return 1;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return 1; }
+struct hipFuncAttributes {
+  int maxThreadsPerBlock;
+};
+inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) {
+  a->maxThreadsPerBlock = 1024;
+  return hipSuccess;
+}
+
 #define FK_LAUNCH(fn, grid, block, stream, ...) \
   emu_launch((grid), (block), [&]() { fn(__VA_ARGS__); })

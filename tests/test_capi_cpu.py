@@ -626,3 +626,63 @@ def test_emu_mfma_form_option_errors(emu_lib):
     with pytest.raises(capi.FFT3DError):
         pl.set_option(capi.Plane.OPT_DFT, 3)
     pl.close()
+
+
+def _raw_step(lib, reqs):
+    """fft3d_step over (plane, frame, nframes, clip, out, on_device) with
+    numpy buffers: in the emulation, device memory is host memory, so the
+    device flags take the batched path on the same buffers."""
+    L = capi.load(lib)
+    keep = []
+    items = (capi.WorkItem * len(reqs))()
+    for k, (pl, f, nf, clip, out, dev) in enumerate(reqs):
+        def cb(_user, fn, c=clip):
+            a = np.ascontiguousarray(c[fn])
+            keep.append(a)
+            return a.ctypes.data
+        c = capi.SOURCE_FN(cb)
+        keep.append(c)
+        items[k] = capi.WorkItem(pl.h, f, nf, c, None, int(dev), out.ctypes.data, out.nbytes,
+                                 int(dev), None, 0)
+    L.fft3d_step(items, len(reqs))
+    return [items[k].status for k in range(len(reqs))]
+
+
+def test_emu_batched_step_equals_item_step(emu_lib):
+    """The batched step (device buffers: one launch per kernel class over all
+    items of a round, each plane's items in rounds) gives the frames of the
+    item-by-item step (host buffers), bit for bit, over planes of several
+    block sizes, temporal modes, ring precisions and DFT forms, with two
+    frames of one plane in the same call."""
+    n = 5
+    cfgs = [(96, 64, 8, dict(bt=4, sharpen=0.4), {}),
+            (80, 72, 10, dict(bt=3, bw=16, bh=16), {}),
+            (128, 96, 8, dict(bt=0, sigma2=3.0, bw=64, bh=64), {}),
+            (96, 80, 8, dict(bt=2), {"half": 1}),
+            (96, 64, 10, dict(bt=-1, sharpen=0.5), {}),
+            (96, 64, 8, dict(bt=4), {"dft": 1})]
+    res = {}
+    for dev in (False, True):
+        planes, clips, outs = [], [], []
+        for c, (W, H, bits, kw, opt) in enumerate(cfgs):
+            pl = capi.Plane(capi.make_params(**kw), W, H, bits, lib=emu_lib)
+            if opt.get("half"):
+                pl.set_option(capi.Plane.OPT_SPECTRA_FP16, 1)
+            if opt.get("dft"):
+                pl.set_option(capi.Plane.OPT_DFT, opt["dft"])
+            planes.append(pl)
+            clips.append(make_clip(60 + c, W, H, n, "u8" if bits == 8 else "u10")[0])
+            outs.append(np.zeros_like(clips[-1]))
+        for f0 in (0, 2, 4):
+            frames = [f for f in (f0, f0 + 1) if f < n]
+            reqs = [(planes[c], f, n, clips[c], outs[c][f], dev) for f in frames
+                    for c in range(len(cfgs))]
+            assert _raw_step(emu_lib, reqs) == [0] * len(reqs)
+        for c, pl in enumerate(planes):
+            bt = cfgs[c][2 + 1]["bt"]
+            assert pl.info.cached_frames <= (bt + 2 if bt >= 2 else 0)
+            pl.close()
+        res[dev] = outs
+    for c in range(len(cfgs)):
+        assert np.array_equal(res[True][c], res[False][c]), cfgs[c]
+        assert res[True][c].any()
