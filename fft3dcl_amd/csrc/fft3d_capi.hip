@@ -1278,7 +1278,12 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   pl->fbw = bw / 2 + 1;
   pl->bps = desc->sample_type == 1 ? 4 : (desc->bits <= 8 ? 1 : 2);
   pl->pix = desc->sample_type == 1 ? 0 : (desc->bits <= 8 ? 8 : 16);
-  pl->specBytes = sizeof(fk::c32) * (size_t)pl->nb * bh * (bw / 2);
+  // precision 0 and 1 (8-bit clips only: deeper ones are forced to 2,
+  // src/params.h:116-120) store the spectra ring as FP16, the original
+  // FFT3dGPU's storage at those settings; 2 keeps FP32. Arithmetic is FP32
+  // either way. Block shapes without FP16-ring kernels keep FP32.
+  pl->half = p.precision < 2 && pl->ks.analysisH != nullptr;
+  pl->specBytes = (pl->half ? 4 : 8) * (size_t)pl->nb * bh * (bw / 2);
   pl->t = fft3d::buildTables(p);
 
   float chromaNorm = 0.0f;

@@ -101,11 +101,11 @@ def footprint(spec: StreamSpec) -> int:
 def form_options(form: str, spec: StreamSpec) -> dict:
     """Instance options of a served form for one stream."""
     if form == "fp32":
-        return {}
+        return {"precision": 2}  # FP32 ring also for 8-bit streams
     if form != "fp16_mfma":
         raise ValueError(f"unknown served form {form!r}")
     bw, bh = spec.params["bw"], spec.params["bh"]
-    opts = {}
+    opts = {"precision": 0}
     if bw == bh and bw in (16, 32, 64):
         opts["half"] = True
     if bw == bh == 32:
@@ -121,7 +121,8 @@ class Server:
 
     def __init__(self, specs, sources, outputs, budget: int, lib=None, on_device=True,
                  batched: bool = False, form: str = "fp32"):
-        """form: "fp32" (FP32 register FFTs and spectra ring) or "fp16_mfma"
+        """form: "fp32" (FP32 register FFTs and spectra ring, also for 8-bit
+        streams, whose default precision=0 would store FP16) or "fp16_mfma"
         (FP16 spectra ring on every square 16/32/64 plane, and the MFMA DFT
         with FP16 hi+lo operands on the 32x32 ones)."""
         self.batched = batched

@@ -279,7 +279,7 @@ def test_emu_sweep_seams_match_oracle(oracle, emu_lib, kind, kw, W, H, R):
 
 
 @pytest.mark.parametrize("kind,kw,W,H", [
-    ("u8", dict(bt=3), 300, 90),
+    ("u8", dict(bt=3, precision=2), 300, 90),  # FP32 ring: the fused form runs
     ("u10", dict(bt=4, sharpen=0.4), 270, 70),
     ("f32", dict(bt=0, kratio=1.5), 200, 60),
     ("u8", dict(bt=1, bw=16, bh=64), 300, 140),           # lanes sharing a column
@@ -477,7 +477,8 @@ def test_emu_staging_options(emu_lib):
 
 def test_emu_forward_process_errors(emu_lib):
     planes = make_clip(15, 96, 64, 4, "u8")
-    pl = capi.Plane(capi.make_params(bt=3), 96, 64, 8, lib=emu_lib)
+    # the fused analysis runs on the FP32 ring only (8-bit default precision 0: FP16)
+    pl = capi.Plane(capi.make_params(bt=3, precision=2), 96, 64, 8, lib=emu_lib)
     pl.set_option(capi.Plane.OPT_FUSED_ANALYSIS, 1)
     with pytest.raises(capi.FFT3DError, match="must be 0 or 1"):
         pl.set_option(capi.Plane.OPT_FUSED_ANALYSIS, 2)
@@ -605,7 +606,7 @@ def test_emu_mfma_form_within_parity(oracle, emu_lib, dft, kind, kw):
             assert max_abs(got[0], want[0]) < 1e-3
         else:
             assert lsb_diff(got[0], want[0]) <= (2 if kind == "u8" else 4)
-    pl = capi.Plane(capi.make_params(**kw), W, H, bits, st, lib=emu_lib)
+    pl = capi.Plane(capi.make_params(**kw, precision=2), W, H, bits, st, lib=emu_lib)
     pl.set_option(capi.Plane.OPT_DFT, dft)
     pl.forward(0, np.ascontiguousarray(planes[0][0]))
     got = pl.read_spectrum(0)

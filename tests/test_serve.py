@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from fft3dcl_amd.filter import GRAY, Instance
-from fft3dcl_amd.serve import Server, footprint, make_specs, summarize
+from fft3dcl_amd.serve import Server, footprint, form_options, make_specs, summarize
 from fft3dcl_amd.synth import make_plane_sequence
 
 
@@ -23,8 +23,9 @@ def _clips(specs, on_device):
     return clips
 
 
-def _alone(sp, clip, lib, on_device):
-    inst = Instance(GRAY, sp.width, sp.height, sp.bits, 0, sp.nframes, lib=lib, **sp.params)
+def _alone(sp, clip, lib, on_device, form="fp32"):
+    inst = Instance(GRAY, sp.width, sp.height, sp.bits, 0, sp.nframes, lib=lib,
+                    **form_options(form, sp), **sp.params)
     if on_device:
         import torch
         out = torch.zeros_like(clip)
