@@ -45,7 +45,8 @@ class PlaneInfo(C.Structure):
                 ("bt", C.c_int), ("current", C.c_int), ("ncols", C.c_int),
                 ("nrows", C.c_int), ("blocks_per_wg", C.c_int),
                 ("spectrum_bytes", C.c_size_t), ("device_bytes", C.c_size_t),
-                ("cached_frames", C.c_int), ("dft", C.c_int), ("spectra_fp16", C.c_int)]
+                ("cached_frames", C.c_int), ("dft", C.c_int), ("spectra_fp16", C.c_int),
+                ("band_rows", C.c_int), ("sweep_occupancy", C.c_int), ("tuned", C.c_int)]
 
 
 class KernelTime(C.Structure):
@@ -302,6 +303,7 @@ class Plane:
     OPT_SPECTRA_FP16 = 5
     OPT_HOST_STAGING, OPT_STAGING_CHUNK = 6, 7
     OPT_DFT = 8  # 0 register FFTs, 1 MFMA with FP16 hi+lo operands, 2 with BF16
+    OPT_SWEEP_OCC = 9  # 0 tuned, 2..4 sweep workgroups per CU
 
     def set_option(self, option: int, value: int):
         self._check(self.lib.fft3d_plane_set_option(self.h, option, int(value)))

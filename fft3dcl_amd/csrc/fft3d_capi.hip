@@ -11,8 +11,12 @@
 #include "fft3d_hip.h"
 #include "fk_xfer.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -131,6 +135,73 @@ bool sepMatches(const fft3d::Tables& t, const fft3d::Params& p) {
   return true;
 }
 
+// ---- tuning table ----
+// Launch parameters chosen by measurement (fft3dcl_amd/tune.py) per block
+// shape, plane size, ring precision and DFT form: sweep band rows and sweep
+// workgroups per CU. Read once from $FFT3D_TUNING, else tuning.json next to
+// this library; planes without an entry use the heuristic.
+struct TuneKey {
+  int bw, bh, w, h, half, dft;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(bw, bh, w, h, half, dft) < std::tie(o.bw, o.bh, o.w, o.h, o.half, o.dft);
+  }
+};
+struct TuneVal {
+  int bandRows, occ;
+};
+std::map<TuneKey, TuneVal> g_tune;
+std::string g_tunePath;
+std::once_flag g_tuneOnce;
+
+// Integer value of "key" in one flat JSON object, or def.
+int json_int(const std::string& obj, const char* key, int def) {
+  const std::string k = std::string("\"") + key + "\"";
+  size_t i = obj.find(k);
+  if (i == std::string::npos) return def;
+  i = obj.find(':', i + k.size());
+  if (i == std::string::npos) return def;
+  char* end = nullptr;
+  const long v = std::strtol(obj.c_str() + i + 1, &end, 10);
+  return end == obj.c_str() + i + 1 ? def : (int)v;
+}
+
+void load_tuning() {
+  const char* env = std::getenv("FFT3D_TUNING");
+  std::string path;
+  if (env && *env) {
+    path = env;
+  } else {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&load_tuning), &info) && info.dli_fname) {
+      path = info.dli_fname;
+      const size_t slash = path.rfind('/');
+      path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+             "/tuning.json";
+    }
+  }
+  FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "rb");
+  if (!f) return;
+  std::string txt;
+  char buf[4096];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) txt.append(buf, n);
+  std::fclose(f);
+  const size_t list = txt.find("\"entries\"");
+  if (list == std::string::npos) return;
+  size_t i = txt.find('[', list);
+  while (i != std::string::npos) {
+    const size_t a = txt.find('{', i), b = a == std::string::npos ? a : txt.find('}', a);
+    if (b == std::string::npos) break;
+    const std::string obj = txt.substr(a, b - a + 1);
+    TuneKey k{json_int(obj, "bw", 0), json_int(obj, "bh", 0), json_int(obj, "width", 0),
+              json_int(obj, "height", 0), json_int(obj, "half", 0), json_int(obj, "dft", 0)};
+    TuneVal v{json_int(obj, "band_rows", 0), json_int(obj, "occupancy", 0)};
+    if (k.bw > 0 && k.bh > 0 && k.w > 0 && k.h > 0) g_tune[k] = v;
+    i = b + 1;
+  }
+  g_tunePath = path;
+}
+
 } // namespace
 
 struct fft3d_plane {
@@ -181,6 +252,7 @@ struct fft3d_plane {
   bool fuseOn = false;       // option: forward_process analyses inside the sweep
   bool half = false;         // option: spectra stored as FP16 pairs
   int dft = 0;               // option: 0 register FFTs, 1/2 MFMA DFT with FP16/BF16 operands
+  int occOpt = 0;            // option: sweep workgroups per CU (0: tuned or 2)
   void* dDft = nullptr;      // the MFMA form's constant operands (fk_mfma.h)
   float* dPart = nullptr;    // 4 partial planes
   fk::SeamTile* dSeams = nullptr;  // border tiles of the current band layout
@@ -393,14 +465,37 @@ struct fft3d_plane {
   }
 
   // Chooses the band height and rebuilds the border-tile list for it.
+  // The tuning table's entry for the current options, if any.
+  const TuneVal* tuned() const {
+    std::call_once(g_tuneOnce, load_tuning);
+    auto it = g_tune.find(TuneKey{p.bw, p.bh, d.width, d.height, half ? 1 : 0, dft});
+    return it == g_tune.end() ? nullptr : &it->second;
+  }
+  // Sweep workgroups per CU: the option, else the table, else 2 (register
+  // FFTs only; the MFMA sweep has one variant).
+  int occupancy() const {
+    int o = occOpt;
+    if (o <= 0) {
+      const TuneVal* t = tuned();
+      o = t && t->occ > 0 ? t->occ : 2;
+    }
+    if (dft || o > ks.maxOcc || o < 2 || !ks.sweepOcc[o - 2][half ? 1 : 0]) o = 2;
+    return o;
+  }
   // Block rows per sweep band for the current options.
   int bandRowsFor() const {
     const int nstrips = (ncols + ks.nbw - 1) / ks.nbw;
     int R = bandRows;
     if (R <= 0) {
+      const TuneVal* t = tuned();
+      if (t && t->bandRows > 0) R = t->bandRows;
+    }
+    if (R <= 0) {
       // one round of workgroups: as many as the device's CUs hold at once,
       // at least 2 rows per band
-      const int slots = (ncu > 0 ? ncu : 256) * std::max(1, ks.sweepPerCU);
+      const int occ = occupancy();
+      const int perCU = occ > 2 ? occ : ks.sweepPerCU;
+      const int slots = (ncu > 0 ? ncu : 256) * std::max(1, perCU);
       int nbands = (slots + nstrips - 1) / nstrips;
       R = (nrows + nbands - 1) / nbands;
       if (R < 2) R = 2;
@@ -593,7 +688,7 @@ const char* fft3d_plane::launch(const fk::WindowArgs& w, void* outp, hipStream_t
     a.fwdPlane = fwdPlane;
     a.fwdSpec = fwdSpec;
     a.fwdMask = fwdMask;
-    SweepFn ksw = fwdMask ? ks.sweepFwd : (half ? ks.sweepH : ks.sweep);
+    SweepFn ksw = fwdMask ? ks.sweepFwd : ks.sweepOcc[occupancy() - 2][half ? 1 : 0];
     if (dft && !fwdMask) ksw = ks.sweepM[dft - 1][half ? 1 : 0];
     hipEvent_t t0 = begin();
     FK_LAUNCH(ksw, dim3(a.nstrips, a.nbands), dim3(ks.sweepThreads), s, args, w, dKalman, kfirst,
@@ -782,7 +877,8 @@ bool step_batchable(const fft3d_work_item& it, int dev) {
   const fft3d_plane* p = it.plane;
   const int hf = p->half ? 1 : 0;
   return it.src_on_device && it.dst_on_device && p->dev == dev && p->sweepOk && p->sweepOn &&
-         p->pendFn < 0 && p->ks.analysisB[p->dft][hf] && p->ks.sweepB[p->dft][hf];
+         p->pendFn < 0 && p->ks.analysisB[p->dft][hf] && p->ks.sweepB[p->dft][hf] &&
+         (p->dft || p->ks.sweepBOcc[p->occupancy() - 2][hf]);
 }
 
 // Process-wide state of the batched step: its stream, the descriptor buffer
@@ -985,7 +1081,7 @@ void step_batch(fft3d_work_item* items, const std::vector<std::vector<int>>& rou
       for (int k : ks) items[k].plane->release_held();
     }
   } release{items, all_items};
-  std::vector<fk::BatchFn> classes;            // by sweep kernel
+  std::vector<std::pair<fk::BatchFn, int>> classes;  // sweep kernel, plane size range
   std::vector<std::vector<ClassRound>> plan;   // [class][round]
   std::vector<std::pair<fft3d_plane*, int>> added;  // ring entries the analyses fill
   for (size_t r = 0; r < rounds.size(); r++) {
@@ -1032,10 +1128,17 @@ void step_batch(fft3d_work_item* items, const std::vector<std::vector<int>>& rou
         continue;
       }
       const int hf = p->half ? 1 : 0;
-      const fk::BatchFn sfn = p->ks.sweepB[p->dft][hf];
-      size_t c = std::find(classes.begin(), classes.end(), sfn) - classes.begin();
+      const fk::BatchFn sfn =
+          p->dft ? p->ks.sweepB[p->dft][hf] : p->ks.sweepBOcc[p->occupancy() - 2][hf];
+      // a class is one sweep kernel and one plane size range: a round of a
+      // class ends with its largest plane, so large and small planes go on
+      // separate streams and small ones are not held back by large ones
+      const long px = (long)p->d.width * p->d.height;
+      const int bucket = px <= (1l << 20) ? 0 : (px <= (3l << 20) ? 1 : 2);
+      const std::pair<fk::BatchFn, int> key{sfn, bucket};
+      size_t c = std::find(classes.begin(), classes.end(), key) - classes.begin();
       if (c == classes.size()) {
-        classes.push_back(sfn);
+        classes.push_back(key);
         plan.emplace_back();
       }
       if (plan[c].size() <= r) plan[c].resize(r + 1);
@@ -1696,6 +1799,14 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
     p->args.dft = value ? reinterpret_cast<const uint4*>(p->dDft) + (value - 1) * slots : nullptr;
     return FFT3D_OK;
   }
+  case FFT3D_OPT_SWEEP_OCC:
+    if (value != 0 && (value < 2 || value > 4))
+      return p->fail(FFT3D_EINVAL, "sweep occupancy must be 0 (tuned) or 2..4");
+    if (value > p->ks.maxOcc)
+      return p->fail(FFT3D_EUNSUPPORTED, "no sweep kernel for that occupancy at this block size");
+    p->occOpt = value;
+    if (p->sweepOk && !p->layoutSweep()) return p->fail(FFT3D_ENOMEM, "GPU alloc failed (seam list)");
+    return FFT3D_OK;
   case FFT3D_OPT_FUSED_ANALYSIS:
     if (value != 0 && value != 1)
       return p->fail(FFT3D_EINVAL, "fused analysis option must be 0 or 1");
@@ -1787,6 +1898,9 @@ int fft3d_plane_get_info(const fft3d_plane* p, fft3d_plane_info* i) {
   i->cached_frames = (int)p->ring.size();
   i->dft = p->dft;
   i->spectra_fp16 = p->half ? 1 : 0;
+  i->band_rows = p->sweepOk ? p->bandRowsFor() : 0;
+  i->sweep_occupancy = p->sweepOk ? p->occupancy() : 0;
+  i->tuned = p->tuned() ? 1 : 0;
   return FFT3D_OK;
 }
 

@@ -56,9 +56,9 @@ __global__ __launch_bounds__(256) void k_analysis_mfma_batch(const BatchItem* __
   analysis_mfma_body<PR, H>(P, it->plane, it->spec, local, it->nwg);
 }
 
-template <int BW, int BH, int WPW, bool H, int DFT>
-__global__ __launch_bounds__(64 * WPW, 2) void k_sweep_batch(const BatchItem* __restrict__ items,
-                                                           int n) {
+template <int BW, int BH, int WPW, bool H, int DFT, int MINB = 2>
+__global__ __launch_bounds__(64 * WPW, MINB) void k_sweep_batch(
+    const BatchItem* __restrict__ items, int n) {
   const BatchItem* it = items + batch_item(items, n, blockIdx.x);
   const int local = blockIdx.x - it->wg0;
   if (local >= it->nwg) return;

@@ -30,6 +30,12 @@ struct KernelSet {
   // batched launches (fft3d_step): [DFT form 0..2][FP16 ring]
   BatchFn analysisB[3][2] = {};
   BatchFn sweepB[3][2] = {};
+  // the register-FFT sweep sized for more resident workgroups per CU:
+  // [workgroups per CU - 2][FP16 ring], where the LDS holds them (square 8
+  // and 16 blocks); [0] is the default sweep
+  SweepFn sweepOcc[3][2] = {};
+  BatchFn sweepBOcc[3][2] = {};
+  int maxOcc = 2;  // largest sweep workgroups per CU with a kernel
   int G = 0;
   int sweepThreads = 0;  // 64 * waves per sweep workgroup
   int nbw = 0;           // block columns per sweep strip

@@ -48,6 +48,24 @@ KernelSet make_set() {
     k.sweepB[2][0] = &k_sweep_batch<BW, BH, WPW, false, mm::BF16>;
     k.sweepB[2][1] = &k_sweep_batch<BW, BH, WPW, true, mm::BF16>;
   }
+  k.sweepOcc[0][0] = k.sweep;
+  k.sweepOcc[0][1] = k.sweepH;
+  k.sweepBOcc[0][0] = k.sweepB[0][0];
+  k.sweepBOcc[0][1] = k.sweepB[0][1];
+  if constexpr (BW == BH && (BW == 8 || BW == 16) && BY_LDS >= 3) {
+    k.sweepOcc[1][0] = &k_filter_synth<BW, BH, WPW, false, false, 0, 3>;
+    k.sweepBOcc[1][0] = &k_sweep_batch<BW, BH, WPW, false, 0, 3>;
+    if constexpr (BW >= 16) {
+      k.sweepOcc[1][1] = &k_filter_synth<BW, BH, WPW, false, true, 0, 3>;
+      k.sweepBOcc[1][1] = &k_sweep_batch<BW, BH, WPW, true, 0, 3>;
+    }
+    k.maxOcc = 3;
+  }
+  if constexpr (BW == BH && BW == 8 && BY_LDS >= 4) {
+    k.sweepOcc[2][0] = &k_filter_synth<BW, BH, WPW, false, false, 0, 4>;
+    k.sweepBOcc[2][0] = &k_sweep_batch<BW, BH, WPW, false, 0, 4>;
+    k.maxOcc = 4;
+  }
   k.G = Shape<BW, BH>::G;
   k.sweepThreads = 64 * WPW;
   k.nbw = SweepShape<BW, BH, WPW>::NBW;

@@ -349,10 +349,13 @@ __device__ __forceinline__ void sweep_body(const PlaneArgs& P, const WindowArgs&
   }
 }
 
-template <int BW, int BH, int WPW, bool FWD, bool H, int DFT = 0>
-__global__ __launch_bounds__(64 * WPW, 2) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
-                                                           float4* kstate, int kfirst,
-                                                           SweepArgs A) {
+// MINB: workgroups per CU the register allocation is sized for (its VGPR
+// budget is 512 / (MINB * WPW / 4) per lane; more resident waves hide more
+// latency when the LDS holds them).
+template <int BW, int BH, int WPW, bool FWD, bool H, int DFT = 0, int MINB = 2>
+__global__ __launch_bounds__(64 * WPW, MINB) void k_filter_synth(PlaneArgs P, WindowArgs Wn,
+                                                              float4* kstate, int kfirst,
+                                                              SweepArgs A) {
   sweep_body<BW, BH, WPW, FWD, H, DFT>(P, Wn, kstate, kfirst, A, blockIdx.x, blockIdx.y);
 }
 

@@ -75,6 +75,9 @@ typedef struct fft3d_plane_info {
   int cached_frames;     /* spectra currently in the ring */
   int dft;               /* DFT form (FFT3D_OPT_DFT) */
   int spectra_fp16;      /* 1: the ring stores FP16 spectra */
+  int band_rows;         /* fused sweep: block rows per band (0 off the sweep) */
+  int sweep_occupancy;   /* fused sweep: workgroups per CU it is built for */
+  int tuned;             /* 1: band rows / occupancy come from the tuning table */
 } fft3d_plane_info;
 
 /* Fills the reference's defaults for a clip with no keys set (degrid 1, ow/oh
@@ -274,6 +277,16 @@ int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max);
  *                        reference's radix-2 fft_batch stages
  *                        (src/cl/kernels.h:84-129); not exact (error stated
  *                        per configuration by the tests and the bench).
+ *   FFT3D_OPT_SWEEP_OCC  0 (default): the tuning table's choice, else 2;
+ *                        2..4: the register-FFT sweep built for that many
+ *                        resident workgroups per CU (square 8 and 16 blocks
+ *                        have 3, 8x8 also 4; FFT3D_EUNSUPPORTED otherwise).
+ * FFT3D_OPT_BAND_ROWS 0 and FFT3D_OPT_SWEEP_OCC 0 take the plane's entry in
+ * the tuning table (block size, plane size, ring precision, DFT form), read
+ * once per process from $FFT3D_TUNING or tuning.json beside the library and
+ * written by measurement (python -m fft3dcl_amd.tune). The reference sizes
+ * its work-groups by formula (src/cl/gpu.cpp:70-81). Results do not depend
+ * on these choices beyond floating-point summation order at band seams.
  * Returns FFT3D_EINVAL for an unknown option or value. */
 enum {
   FFT3D_OPT_SWEEP = 1,
@@ -283,7 +296,8 @@ enum {
   FFT3D_OPT_SPECTRA_FP16 = 5,
   FFT3D_OPT_HOST_STAGING = 6,
   FFT3D_OPT_STAGING_CHUNK = 7,
-  FFT3D_OPT_DFT = 8
+  FFT3D_OPT_DFT = 8,
+  FFT3D_OPT_SWEEP_OCC = 9
 };
 int fft3d_plane_set_option(fft3d_plane* p, int option, int value);
 

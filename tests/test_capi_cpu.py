@@ -687,3 +687,42 @@ def test_emu_batched_step_equals_item_step(emu_lib):
     for c in range(len(cfgs)):
         assert np.array_equal(res[True][c], res[False][c]), cfgs[c]
         assert res[True][c].any()
+
+
+def test_emu_tuning_table_is_read(emu_lib, tmp_path):
+    """The library reads the tuning table named by $FFT3D_TUNING once per
+    process and applies an entry to the plane that matches it exactly (block
+    size, plane size, ring precision, DFT form); an occupancy the block size
+    has no kernel for falls back to 2; other planes keep the heuristic."""
+    import json
+    import subprocess
+    import sys
+    tab = {"entries": [
+        {"bw": 16, "bh": 16, "width": 320, "height": 200, "half": 0, "dft": 0,
+         "band_rows": 3, "occupancy": 3},
+        {"bw": 32, "bh": 32, "width": 320, "height": 200, "half": 0, "dft": 0,
+         "band_rows": 2, "occupancy": 4},
+    ]}
+    path = tmp_path / "tuning.json"
+    path.write_text(json.dumps(tab))
+    code = f"""
+import sys
+sys.path.insert(0, {str(ROOT)!r})
+from fft3dcl_amd import capi
+lib = {emu_lib!r}
+def info(**kw):
+    p = capi.Plane(capi.make_params(bt=3, precision=2, **kw), kw.pop('w', 320), 200, 8, lib=lib)
+    i = p.info
+    r = (i.tuned, i.band_rows, i.sweep_occupancy)
+    p.close()
+    return r
+print(info(bw=16, bh=16), info(bw=32, bh=32), info(bw=8, bh=8))
+"""
+    env = dict(__import__("os").environ, FFT3D_TUNING=str(path))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    a, b, c = eval(r.stdout.strip().replace(") (", "),("))
+    assert a == (1, 3, 3)
+    assert b == (1, 2, 2)      # no occupancy-4 sweep for 32x32 blocks
+    assert c[0] == 0 and c[2] == 2

@@ -503,3 +503,41 @@ def test_mfma_form_option_errors(gpu):
             pl.set_option(capi.Plane.OPT_DFT, 1)
         assert e.value.code == -6
         pl.close()
+
+
+def _tuned_oracle(args):
+    """Oracle frames 1-3 of a tuning workload (worker process)."""
+    w, h, bw = args
+    from oracle_lib import Oracle, script
+    clip = make_clip(300 + bw, w, h, 5, "u10")
+    out, _ = Oracle().run_clip(script(sigma=3.0, bt=4, sharpen=0.4, bw=bw, bh=bw, ow=bw // 2,
+                                      oh=bw // 2), clip, 10, 0, order=[1, 2, 3])
+    return (w, h, bw), out[0][1:4]
+
+
+def test_tuning_table_choices_within_parity(gpu):
+    """Every entry of the committed tuning table (fft3dcl_amd/tuning.json): a
+    plane of that block size, plane size, ring precision and DFT form picks
+    the entry's band rows and sweep occupancy, and its output stays within
+    the parity bar of the oracle (+/-4 LSB at 10 bit)."""
+    import json
+    import multiprocessing as mp
+    from pathlib import Path
+    tab = json.loads((Path(capi.PKG) / "tuning.json").read_text())["entries"]
+    keys = sorted({(e["width"], e["height"], e["bw"]) for e in tab})
+    with mp.get_context("spawn").Pool(8) as pool:
+        want = dict(pool.map(_tuned_oracle, keys))
+    for e in tab:
+        w, h, bw = e["width"], e["height"], e["bw"]
+        clip = make_clip(300 + bw, w, h, 5, "u10")
+        inst = Instance(1, w, h, 10, 0, 5, lib=gpu, half=bool(e["half"]), dft=e["dft"],
+                        sigma=3.0, bt=4, sharpen=0.4, bw=bw, bh=bw, ow=bw // 2, oh=bw // 2)
+        info = inst.objs[0].info
+        assert info.tuned == 1 and info.band_rows == e["band_rows"], e
+        assert info.sweep_occupancy == e["occupancy"], e
+        out = np.zeros_like(clip[0])
+        for n in range(1, 4):
+            inst.get_frame(n, [lambda fn: np.ascontiguousarray(clip[0][fn])], [out[n]])
+        inst.close()
+        d = lsb_diff(out[1:4], want[(w, h, bw)])
+        assert d <= 4, (e, d)
