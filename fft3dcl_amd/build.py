@@ -48,8 +48,10 @@ def build_hip(out: Path = HIP_LIB) -> Path:
         units += [(CSRC / "fk_set_bw.hip", [f"-DFK_BW={bw}", f"-DFK_BH={bh}"],
                    Path(tmpdir) / f"s{bw}x{bh}.o")
                   for bw in BLOCK_SIZES for bh in BLOCK_SIZES]
+        # the shapes with a side above 64 (LDS FFTs), one unit
+        units.append((CSRC / "fk_set_large.hip", ["-DFK_LARGE_SET=1"], Path(tmpdir) / "large.o"))
         # largest shapes first: they take longest
-        units.sort(key=lambda u: -sum(int(d.split("=")[1]) for d in u[1]))
+        units.sort(key=lambda u: -sum(int(d.split("=")[1]) for d in u[1]))  # longest first
 
         def compile_unit(u):
             src, d, o = u

@@ -75,13 +75,15 @@ KernelSet kernels_for_bh(int bh) {
 
 KernelSet kernels_for(int bw, int bh) {
   switch (bw) {
-  case 4: return kernels_for_bh<4>(bh);
-  case 8: return kernels_for_bh<8>(bh);
-  case 16: return kernels_for_bh<16>(bh);
-  case 32: return kernels_for_bh<32>(bh);
-  case 64: return kernels_for_bh<64>(bh);
+  case 4: if (bh <= 64) return kernels_for_bh<4>(bh); break;
+  case 8: if (bh <= 64) return kernels_for_bh<8>(bh); break;
+  case 16: if (bh <= 64) return kernels_for_bh<16>(bh); break;
+  case 32: if (bh <= 64) return kernels_for_bh<32>(bh); break;
+  case 64: if (bh <= 64) return kernels_for_bh<64>(bh); break;
   }
-  return KernelSet{};
+  // a side above 64, bw * bh <= 4096 (the reference GPU's 32 KB of block
+  // LDS, src/cl/gpu.cpp:82-87)
+  return fk::kernel_set_large(bw, bh);
 }
 
 void set_err(char* err, size_t n, const std::string& s) {
@@ -656,7 +658,7 @@ struct fft3d_plane {
     } else {
       const int grid = (nb + ks.G - 1) / ks.G;
       AnalysisFn k = half ? ks.analysisH : ks.analysis;
-      FK_LAUNCH(k, dim3(grid), dim3(64), s, args, plane, spec);
+      FK_LAUNCH(k, dim3(grid), dim3(ks.fftThreads), s, args, plane, spec);
     }
     prof_end(t0, K_ANALYSIS, s);
   }
@@ -707,7 +709,7 @@ const char* fft3d_plane::launch(const fk::WindowArgs& w, void* outp, hipStream_t
     const int grid = (nb + ks.G - 1) / ks.G;
     FilterFn kf = half ? ks.filterH : ks.filter_inverse;
     hipEvent_t t0 = begin();
-    FK_LAUNCH(kf, dim3(grid), dim3(64), s, args, w, dKalman, kfirst, blocks);
+    FK_LAUNCH(kf, dim3(grid), dim3(ks.fftThreads), s, args, w, dKalman, kfirst, blocks);
     prof_end(t0, K_FILTER, s);
     if (!ok()) return "filter/inverse";
     if (p.bm == 0) kalmanFirst = false;
@@ -1369,7 +1371,8 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   }
   if (!pl->ks.analysis) {
     delete pl;
-    set_err(err, errlen, "FFT3DCL: GPU setup failed: block too large for the GPU path (max 64x64)");
+    set_err(err, errlen,
+            "FFT3DCL: GPU setup failed: block too large for device local memory (max 64x64)");
     return FFT3D_EUNSUPPORTED;
   }
   ++g_live;
@@ -1403,7 +1406,15 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   const size_t nwin = (size_t)bw * bh, nbin = (size_t)pl->fbw * bh;
   const size_t sep = 3 * nwin + 4 * nbin;
   const size_t fac = (sep + 2 * bw + 2 * bh + 3) / 4 * 4;  // float4 aligned
-  std::vector<float> host(fac + 4 * (size_t)bh + 4 * (size_t)pl->fbw, 0.0f);
+  // twiddles of the LDS FFTs (sides above 64): (cos, sin)(2 pi k / nmax)
+  const int nmax = std::max(bw, bh);
+  const size_t twOff = fac + 4 * (size_t)bh + 4 * (size_t)pl->fbw;
+  std::vector<float> host(twOff + (size_t)nmax, 0.0f);
+  for (int k = 0; k < nmax / 2; k++) {
+    const double a = 2.0 * 3.14159265358979323846 * k / nmax;
+    host[twOff + 2 * k] = (float)std::cos(a);
+    host[twOff + 2 * k + 1] = (float)std::sin(a);
+  }
   std::copy(pl->t.sepRow.begin(), pl->t.sepRow.end(), host.begin() + fac);
   std::copy(pl->t.sepCol.begin(), pl->t.sepCol.end(), host.begin() + fac + 4 * bh);
   std::copy(pl->t.anaX.begin(), pl->t.anaX.end(), host.begin() + sep);
@@ -1424,7 +1435,8 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   ok = ok && pl->alloc(reinterpret_cast<void**>(&pl->dBlocks), sizeof(float) * (size_t)pl->nb * bw * bh);
   if (ok && p.bm == 0)
     ok = pl->alloc(reinterpret_cast<void**>(&pl->dKalman), sizeof(float4) * (size_t)pl->nb * nbin);
-  pl->sweepOk = p.mode == 1 && p.ow * 2 == bw && p.oh * 2 == bh && sepMatches(pl->t, p);
+  pl->sweepOk = pl->ks.sweep && p.mode == 1 && p.ow * 2 == bw && p.oh * 2 == bh &&
+                sepMatches(pl->t, p);
   if (ok && pl->sweepOk) {
     ok = pl->alloc(reinterpret_cast<void**>(&pl->dPart),
                    sizeof(float) * 4 * (size_t)((W + 3) & ~3) * H);
@@ -1489,6 +1501,7 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   a.synY = pl->dTables + sep + 2 * bw + bh;
   a.sepRow = reinterpret_cast<const float4*>(pl->dTables + fac);
   a.sepCol = reinterpret_cast<const float4*>(pl->dTables + fac + 4 * bh);
+  a.tw = reinterpret_cast<const float2*>(pl->dTables + twOff);
   *out = pl;
   return FFT3D_OK;
 }

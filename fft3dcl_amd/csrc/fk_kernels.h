@@ -62,6 +62,9 @@ struct PlaneArgs {
   const float4* sepCol;  // bw/2+1
   // constant operands of the MFMA DFT form (fk_mfma.h), null when not used
   const uint4* dft;
+  // twiddles (cos, sin)(2 pi k / max(bw, bh)), k < max(bw, bh) / 2, for the
+  // LDS FFTs of blocks with a side above 64 (fk_large.h)
+  const float2* tw;
 };
 
 // The temporal window handed to the filter: frames in temporal order.

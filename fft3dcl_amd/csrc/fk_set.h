@@ -37,6 +37,7 @@ struct KernelSet {
   BatchFn sweepBOcc[3][2] = {};
   int maxOcc = 2;  // largest sweep workgroups per CU with a kernel
   int G = 0;
+  int fftThreads = 64;   // threads per analysis / filter-inverse workgroup
   int sweepThreads = 0;  // 64 * waves per sweep workgroup
   int nbw = 0;           // block columns per sweep strip
   int sweepPerCU = 0;    // sweep workgroups one CU holds (LDS, 2 waves per SIMD)
@@ -46,5 +47,8 @@ struct KernelSet {
 // each shape is defined by its own compile of fk_set_bw.hip.
 template <int BW, int BH>
 KernelSet kernel_set();
+
+// Shapes with a side above 64 (fk_set_large.hip); empty for others.
+KernelSet kernel_set_large(int bw, int bh);
 
 }  // namespace fk

@@ -59,7 +59,7 @@ def test_params_default_matches_reference_defaults(emu_lib):
     (dict(bt=7), "FFT3DCL: valid modes for bt are -1,0,1,2,3,4"),
     (dict(mode=2), "FFT3DCL: mode 2 is not implemented yet in this port (use mode 0 or 1)"),
     (dict(interlaced=1), "FFT3DCL: interlaced is not implemented yet in this port"),
-    (dict(bw=128), "FFT3DCL: GPU setup failed: block too large for the GPU path (max 64x64)"),
+    (dict(bw=256), "FFT3DCL: GPU setup failed: block too large for device local memory (max 64x64)"),
 ])
 def test_create_errors(emu_lib, kw, msg):
     with pytest.raises(capi.FFT3DError) as e:
@@ -726,3 +726,34 @@ print(info(bw=16, bh=16), info(bw=32, bh=32), info(bw=8, bh=8))
     assert a == (1, 3, 3)
     assert b == (1, 2, 2)      # no occupancy-4 sweep for 32x32 blocks
     assert c[0] == 0 and c[2] == 2
+
+
+@pytest.mark.parametrize("bw,bh,kind,kw", [
+    (128, 32, "u8", dict(bt=3)),
+    (16, 256, "u10", dict(bt=4, sharpen=0.4)),
+    (512, 8, "f32", dict(bt=0, sigma2=3.0)),
+    (8, 512, "u8", dict(bt=2, mode=0)),
+    (256, 16, "u16", dict(bt=-1, sharpen=0.5, wintype=1)),
+])
+def test_emu_large_blocks_match_oracle(oracle, emu_lib, bw, bh, kind, kw):
+    """Blocks with a side above 64 (LDS FFTs, one workgroup per block), as
+    the reference GPU accepts up to bw*bh*8 = 32 KB: spectra to 1e-6 of the
+    largest bin and clips within the parity bar."""
+    bits, st = kind_info(kind)
+    W, H = 300, 160
+    planes = make_clip(3, W, H, 4, kind)
+    kw = dict(kw, bw=bw, bh=bh)
+    got = FFT3DCL(planes, bits=bits, sample_type=st, lib=emu_lib, **kw)
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st)
+    if kind == "f32":
+        assert max_abs(got[0], want[0]) < 1e-3
+    else:
+        assert lsb_diff(got[0], want[0]) <= (2 if kind == "u8" else 4)
+    pl = capi.Plane(capi.make_params(**kw), W, H, bits, st, lib=emu_lib)
+    assert pl.info.blocks_per_wg == 1 and pl.info.band_rows == 0  # unfused path
+    pl.forward(0, np.ascontiguousarray(planes[0][0]))
+    g = pl.read_spectrum(0)
+    op = OPlane(oracle, script(**kw), W, H, bits, st)
+    ref = np.asarray(op.spectra(op.upload(planes[0][0]))).reshape(g.shape)
+    assert np.abs(g - ref).max() < 1e-6 * np.abs(ref).max()
+    pl.close()

@@ -112,6 +112,13 @@ EDGE = [
     ("u8", 160, 128, 4, dict(bt=3, bw=64, bh=16)),
     ("f32", 96, 72, 4, dict(bt=-1, sharpen=1.0)),
     ("u8", 96, 72, 5, dict(bt=0, sigma2=4.0, sigma3=2.0, sigma4=1.0)),
+    # blocks with a side above 64 (LDS FFTs), up to bw*bh = 4096
+    ("u8", 1920, 1080, 4, dict(bt=3, bw=128, bh=32)),
+    ("u10", 700, 500, 5, dict(bt=4, sharpen=0.4, bw=16, bh=256)),
+    ("f32", 600, 300, 4, dict(bt=0, sigma2=3.0, bw=512, bh=8)),
+    ("u8", 500, 600, 4, dict(bt=2, mode=0, bw=8, bh=512)),
+    ("u16", 640, 360, 4, dict(bt=-1, sharpen=0.5, bw=256, bh=16)),
+    ("u8", 333, 201, 4, dict(bt=1, bw=128, bh=8, wintype=2)),
 ]
 
 

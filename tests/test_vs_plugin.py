@@ -56,7 +56,7 @@ def test_shipped_plugin_registration(host):
     (dict(plane=7), "FFT3DCL: plane must be 0 (luma), 1 (chroma) or 4 (all)"),
     (dict(device=0), "FFT3DCL: device=0 selects the CPU backend, which this HIP build does "
                      "not include (use device=1 or 2)"),
-    (dict(bw=200), "FFT3DCL: GPU setup failed: block too large for the GPU path (max 64x64)"),
+    (dict(bw=200), "FFT3DCL: GPU setup failed: block too large for device local memory (max 64x64)"),
 ])
 def test_create_errors(host, kw, msg):
     planes = make_clip(1, 64, 48, 3, "u8")
@@ -111,7 +111,7 @@ def test_prefetch_env(host, monkeypatch, bt, current, pad):
 CASES = [("u8", dict(bt=3), 1, 0), ("u10", dict(bt=4, sharpen=0.4, sigma=3.0), 1, 16),
          ("u8", dict(bt=0), 1, 0), ("f32", dict(bt=2, sigma2=3.0), 1, 8),
          ("u8", dict(plane=4, bt=3), 3, 0), ("u10", dict(plane=1, bt=4), 3, 24),
-         ("u8", dict(bt=1), 2, 0)]
+         ("u8", dict(bt=1), 2, 0), ("u8", dict(bt=3, bw=128, bh=32), 1, 0)]
 
 
 def run_vs_vs_oracle(host, oracle, plugin, kind, kw, cf, pad, W=96, H=64, n=5):
