@@ -236,33 +236,47 @@ __device__ __forceinline__ void load8(const PlaneArgs& P, const void* plane, int
   for (int k = 0; k < 8; k++) z[k] -= P.chromaNorm;
 }
 
-// Windowed 2-D R2C DFT of 32x32 block b by the calling wave (all 64 lanes,
-// wave-uniform b), stored packed into spec as k_analysis does.
-template <int PR, bool H>
-__device__ __forceinline__ void forward_block_mfma(const PlaneArgs& P, const void* plane, int b,
-                                                   c32* spec, int lane) {
-  constexpr int BW = 32, NCOL = 16, BSTR = 32 * 16;
+// The lane's 16 samples of 32x32 block b (row l%32, columns 8h..8h+7 and
+// 16+8h..16+8h+7) times the lane's analysis window factors wf.
+__device__ __forceinline__ void load_block32(const PlaneArgs& P, const void* plane, int b,
+                                             int lane, const float* wf, float* xs) {
   const int h = lane >> 5, li = lane & 31;
   int x0, y0, par;
   block_origin(P, b, x0, y0, par);
   const int y = mirror(y0 + li, P.H);
-  float xs[16];
   load8(P, plane, x0 + 8 * h, y, xs);
   load8(P, plane, x0 + 16 + 8 * h, y, xs + 8);
+#pragma unroll
+  for (int k = 0; k < 16; k++) xs[k] *= wf[k];
+}
+
+// The lane's window factors: the same for every block (mode 1: the separable
+// factors' product anaX[k] * anaY[r], as the register form applies them).
+__device__ __forceinline__ void window32(const PlaneArgs& P, int lane, float* wf) {
+  const int h = lane >> 5, li = lane & 31;
   if (P.mode == 1) {
     const float fy = P.anaY[li];
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
-      for (int j = 0; j < 8; j++) xs[8 * c + j] *= P.anaX[16 * c + 8 * h + j] * fy;
+      for (int j = 0; j < 8; j++) wf[8 * c + j] = P.anaX[16 * c + 8 * h + j] * fy;
   } else {
-    const float* wr = P.winAna + li * BW;
+    const float* wr = P.winAna + li * 32;
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
-      for (int j = 0; j < 8; j++) xs[8 * c + j] *= wr[16 * c + 8 * h + j];
+      for (int j = 0; j < 8; j++) wf[8 * c + j] = wr[16 * c + 8 * h + j];
   }
-  const uint4* T = P.dft;
+}
+
+// 2-D R2C DFT of the windowed 32x32 block b whose samples the wave holds
+// (xs, as load_block32 leaves them) by all 64 lanes, stored packed into spec
+// as k_analysis does. T: the forward constant operands (slots 0-11).
+template <int PR, bool H>
+__device__ __forceinline__ void forward_block_mfma(const float* xs, const uint4* T, int b,
+                                                   c32* spec, int lane) {
+  constexpr int NCOL = 16, BSTR = 32 * 16;
+  const int h = lane >> 5, li = lane & 31;
   V8 ah[2], al[2];
   split8<PR>(xs, ah[0], al[0]);
   split8<PR>(xs + 8, ah[1], al[1]);
@@ -299,17 +313,30 @@ __device__ __forceinline__ void forward_block_mfma(const PlaneArgs& P, const voi
   }
 }
 
+// The inverse constant operands (slots 12-23) of this lane, loaded once for
+// all the blocks a wave transforms in a row.
+struct InvOps {
+  V8 c[4], s[4], n[4];
+};
+__device__ __forceinline__ void load_inv_ops(const uint4* T, int lane, InvOps& o) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    o.c[i] = cop(T, 12 + i, lane);
+    o.s[i] = cop(T, 16 + i, lane);
+    o.n[i] = cop(T, 20 + i, lane);
+  }
+}
+
 // Inverse 2-D C2R of the filtered 32x32 block in its LDS area Yb (packed
 // columns, row pitch `pitch` complex), by the calling wave; the windowed rows
 // (x synX[k] synY[r], factors that include 1/(bw*bh)) are written back over
 // Yb as rows of `op` floats. Contains the wave barrier between the reads of
 // Yb and the writes.
 template <int PR>
-__device__ __forceinline__ void inverse_block_mfma(const PlaneArgs& P, c32* Yb, int pitch, int op,
+__device__ __forceinline__ void inverse_block_mfma(const InvOps& O, c32* Yb, int pitch, int op,
                                                    const float* synX, const float* synY,
                                                    int lane) {
   const int h = lane >> 5, li = lane & 31;
-  const uint4* T = P.dft;
   // A = F^T: lane li is real column li (complex column li/2, component li&1)
   const float* Yf = reinterpret_cast<const float*>(Yb);
   float fa[16];
@@ -323,8 +350,8 @@ __device__ __forceinline__ void inverse_block_mfma(const PlaneArgs& P, c32* Yb, 
   Acc p = acc_zero(), q = acc_zero();
 #pragma unroll
   for (int c = 0; c < 2; c++) {
-    mma3<PR>(p, ah[c], al[c], cop(T, 12 + 2 * c, lane), cop(T, 13 + 2 * c, lane));
-    mma3<PR>(q, ah[c], al[c], cop(T, 16 + 2 * c, lane), cop(T, 17 + 2 * c, lane));
+    mma3<PR>(p, ah[c], al[c], O.c[2 * c], O.c[2 * c + 1]);
+    mma3<PR>(q, ah[c], al[c], O.s[2 * c], O.s[2 * c + 1]);
   }
   // lane (row r = li, h): registers (2s, 2s+1) are (re, im) of a column
   float yp[16];
@@ -338,8 +365,7 @@ __device__ __forceinline__ void inverse_block_mfma(const PlaneArgs& P, c32* Yb, 
   split8<PR>(yp + 8, bh[1], bl[1]);
   Acc x = acc_zero();
 #pragma unroll
-  for (int c = 0; c < 2; c++)
-    mma3<PR>(x, cop(T, 20 + 2 * c, lane), cop(T, 21 + 2 * c, lane), bh[c], bl[c]);
+  for (int c = 0; c < 2; c++) mma3<PR>(x, O.n[2 * c], O.n[2 * c + 1], bh[c], bl[c]);
   wave_sync();  // every lane has read Yb
   const float fy = synY[li];
   float* o = reinterpret_cast<float*>(Yb) + li * op;
@@ -358,15 +384,27 @@ __device__ __forceinline__ void inverse_block_mfma(const PlaneArgs& P, c32* Yb, 
 // workgroup's 32 consecutive blocks (XCD-contiguous, see xcd_contiguous) go
 // 4 at a time to its waves, so neighbouring blocks, which share pixel rows,
 // are read together.
+// The forward constant operands go to LDS once per workgroup; each wave
+// loads the samples of its next block while it transforms the current one.
 template <int PR, bool H>
 __device__ __forceinline__ void analysis_mfma_body(const PlaneArgs& P, const void* plane,
                                                    c32* spec, int wg, int nwg) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ uint4 ctab[12 * 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = tid; i < 12 * 64; i += 256) ctab[i] = P.dft[i];
   const int g = xcd_contiguous(wg, nwg);
-  for (int i = 0; i < 8; i++) {
-    const int b = g * 32 + i * 4 + wv;
-    if (b >= P.nb) break;  // wave-uniform
-    mm::forward_block_mfma<PR, H>(P, plane, b, spec, lane);
+  float wf[16], xs[16], xn[16];
+  mm::window32(P, lane, wf);
+  int b = g * 32 + wv;
+  if (b < P.nb) mm::load_block32(P, plane, b, lane, wf, xs);
+  __syncthreads();  // the constant operands are in LDS
+  for (int i = 0; i < 8 && b < P.nb; i++) {  // b is wave-uniform
+    const int bn = b + 4;
+    if (i + 1 < 8 && bn < P.nb) mm::load_block32(P, plane, bn, lane, wf, xn);
+    mm::forward_block_mfma<PR, H>(xs, ctab, b, spec, lane);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xs[k] = xn[k];
+    b = bn;
   }
 }
 template <int PR, bool H>

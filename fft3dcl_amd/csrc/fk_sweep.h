@@ -173,11 +173,14 @@ __device__ __forceinline__ void sweep_body(const PlaneArgs& P, const WindowArgs&
                                FWD ? A.fwdMask : 0);
     if constexpr (DFT != 0) {
       static_assert(BW == 32 && BH == 32 && !FWD, "the MFMA inverse is for 32x32 blocks");
-      // the wave's blocks one after the other, each by all 64 lanes
+      // the wave's blocks one after the other, each by all 64 lanes, with
+      // the constant operands loaded once for them
+      mm::InvOps ops;
+      mm::load_inv_ops(P.dft, lane, ops);
       for (int gg = 0; gg < S::G; gg++) {
         if (n0 + w * S::G + gg >= P.ncols) break;  // wave-uniform
         c32* Yg = reinterpret_cast<c32*>(lds) + (w * S::G + gg) * S::YS;
-        mm::inverse_block_mfma<DFT>(P, Yg, S::PITCH, T::OP, synX, synY, lane);
+        mm::inverse_block_mfma<DFT>(ops, Yg, S::PITCH, T::OP, synX, synY, lane);
       }
     } else {
       float xr[BW], xi[BW];
