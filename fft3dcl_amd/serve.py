@@ -120,13 +120,17 @@ class Server:
     (sid, n) -> buffer receives the filtered frame."""
 
     def __init__(self, specs, sources, outputs, budget: int, lib=None, on_device=True,
-                 batched: bool = False, form: str = "fp32"):
+                 batched: bool = False, form: str = "fp32", one_queue: bool = False):
         """form: "fp32" (FP32 register FFTs and spectra ring, also for 8-bit
         streams, whose default precision=0 would store FP16) or "fp16_mfma"
         (FP16 spectra ring on every square 16/32/64 plane, and the MFMA DFT
         with FP16 hi+lo operands on the 32x32 ones)."""
         self.batched = batched
         self.form = form
+        # one_queue: every stream's work on one HIP stream, in issue order -
+        # the plain per-clip form (the reference serves all of a process's
+        # clips through one in-order queue, src/cl/cl.cpp:175)
+        self.one_queue = one_queue
         self.specs = {s.sid: s for s in specs}
         self.sources = sources
         self.outputs = outputs
@@ -148,6 +152,7 @@ class Server:
         done_events = []                            # (sid, n, arrival, event)
         dev = self.on_device
         start = torch.cuda.Event(enable_timing=True) if dev else None
+        shared = torch.cuda.Stream() if dev and self.one_queue else None
 
         def admit():
             nonlocal used
@@ -163,7 +168,7 @@ class Server:
                 # the ring's spectra are allocated now, not inside the loop
                 inst.objs[0].set_option(inst.objs[0].OPT_RESERVE_SPECTRA,
                                         max(sp.params["bt"], 1) + 2)
-                st = torch.cuda.Stream() if dev else None
+                st = (shared if self.one_queue else torch.cuda.Stream()) if dev else None
                 admitted[sid] = dict(inst=inst, stream=st, next=0, need=need)
                 used += need
                 self.max_admitted = max(self.max_admitted, len(admitted))
