@@ -495,16 +495,51 @@ __device__ __forceinline__ float4 col_factor(const float4* tab, int bh, int col,
 // repacked W'[r] = H0[r] + i Hh[r], H = (F[r] + conj F[-r]) / 2 the Hermitian
 // part that the real inverse transform keeps.
 // Both passes are called by every lane of the block (lb = lane in block).
+// A stored bin as loaded: complex FP32, or with H the packed FP16 pair
+// (unpacked when the bin is used, so a load in flight holds one register).
+template <bool H>
+struct SpecWord {
+  using T = c32;
+};
+template <>
+struct SpecWord<true> {
+  using T = uint32_t;
+};
+template <bool H>
+__device__ __forceinline__ typename SpecWord<H>::T spec_word(const c32* s, size_t i) {
+  if constexpr (H) return reinterpret_cast<const uint32_t*>(s)[i];
+  else return s[i];
+}
+template <bool H>
+__device__ __forceinline__ c32 spec_value(typename SpecWord<H>::T v) {
+  if constexpr (H) {
+    const float2 u = unpack_h2(v);
+    return cmake(u.x, u.y);
+  } else {
+    return v;
+  }
+}
+
+// Rows per load batch and batch buffers of pass 1: one batch loads while the
+// one before is filtered. Measured on gfx950 at 4K bw=32 (sweep us, FP32 /
+// FP16 ring): 4 rows x 2 buffers 158 / 137; 2 x 4: 162 / 137; 1 x 8: 173 /
+// 137; 4 x 4: 207 (spills) / 138; FP16 2 x 8: 141, 8 x 2: 137. Deeper
+// prefetch does not pay: the sweep is bound by its compute phases.
+constexpr int kPass1Rows = 4, kPass1Buffers = 2;
+
 template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT>
 __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
   using S = Shape<BW, BH>;
+  using Word = typename SpecWord<H>::T;
   constexpr int FBW = S::FBW, NCOL = S::NCOL;
   constexpr int RS = S::LPB / NCOL;  // lanes sharing a column split its rows
   constexpr int NIT = BH / RS;       // rows per lane
-  constexpr int U = NIT < 4 ? NIT : 4;  // rows per batch; divides NIT (powers of two)
-  constexpr int NB = NIT / U;
+  constexpr int U = NIT < kPass1Rows ? NIT : kPass1Rows;  // rows per batch; divides NIT
+  constexpr int NB = NIT / U;                               // batches
+  constexpr int D = NB < kPass1Buffers ? NB : kPass1Buffers;  // batch buffers
+  static_assert(NB % D == 0, "whole rounds of batch buffers");
   const size_t base = (size_t)b * S::BSTR;
   const int c = lb % NCOL, r0 = lb / NCOL;
   float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
@@ -512,42 +547,51 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   if constexpr (SEP) cf = col_factor(tab, BH, c, dc);
   // All four slots are loaded and staged: slots past nwin repeat a window
   // spectrum (the host fills them), which keeps every load unconditional, so
-  // the next batch stays in flight while this one is filtered.
-  auto load = [&](int i0, c32(&va)[U][4]) {
+  // the next batches stay in flight while this one is filtered.
+  auto load = [&](int i0, Word(&va)[U][4]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int r = r0 + (i0 + u) * RS;
 #pragma unroll
-      for (int t = 0; t < 4; t++) va[u][t] = spec_load<H>(Wn.s[t], base + r * NCOL + c);
+      for (int t = 0; t < 4; t++) va[u][t] = spec_word<H>(Wn.s[t], base + r * NCOL + c);
     }
   };
-  auto work = [&](int i0, const c32(&va)[U][4]) {
+  auto work = [&](int i0, const Word(&va)[U][4]) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int r = r0 + (i0 + u) * RS;
+      c32 v[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) v[t] = spec_value<H>(va[u][t]);
       if (c == 0) {
 #pragma unroll
-        for (int t = 0; t < 4; t++) Yb[S::st(t, r)] = va[u][t];
+        for (int t = 0; t < 4; t++) Yb[S::st(t, r)] = v[t];
       } else {
         BinConst kc;
         if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP, PAT>(P, r, c, tab[r], cf);
         else kc = bin_const<BW, DG, SHP, PAT>(P, r, c, dc);
-        Yb[S::yidx(c, r)] = filter_bin<BM, DG, SHP>(P, va[u], kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
+        Yb[S::yidx(c, r)] = filter_bin<BM, DG, SHP>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                        kb ? kb + r * FBW + c : nullptr, kfirst);
       }
     }
   };
-  c32 va[U][4], vb[U][4];
-  load(0, va);
-  if constexpr (NB == 1) {
-    work(0, va);
+  Word buf[D][U][4];
+#pragma unroll
+  for (int d = 0; d + 1 < D; d++) load(d * U, buf[d]);
+  if constexpr (NB == D) {
+    load((D - 1) * U, buf[D - 1]);
+#pragma unroll
+    for (int d = 0; d < D; d++) work(d * U, buf[d]);
   } else {
 #pragma unroll 1
-    for (int k = 0; k < NB; k += 2) {  // NB is even
-      load((k + 1) * U, vb);
-      work(k * U, va);
-      if (k + 2 < NB) load((k + 2) * U, va);
-      work((k + 1) * U, vb);
+    for (int k = 0; k < NB; k += D) {
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        // batch k + d + D - 1 goes into the buffer batch k + d - 1 used
+        const int kn = k + d + D - 1;
+        if (kn < NB) load(kn * U, buf[(d + D - 1) % D]);
+        work((k + d) * U, buf[d]);
+      }
     }
   }
 }
