@@ -244,7 +244,8 @@ struct fft3d_plane {
   hipEvent_t callStart = nullptr;     // start of the call that runs the prefetch
   bool callStarted = false;
   float* dBlocks = nullptr;  // windowed blocks awaiting overlap-add (unfused path)
-  float4* dKalman = nullptr; // bm == 0 state
+  float4* dKalman = nullptr; // bm == 0 state (FP16 pairs with the FP16 ring)
+  size_t kalmanBytes = 0;
   bool kalmanFirst = true;
 
   // fused sweep path
@@ -1433,8 +1434,11 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   ok = ok && pl->alloc(reinterpret_cast<void**>(&pl->dTables), host.size() * sizeof(float));
   ok = ok && pl->alloc(&pl->dStage, (size_t)W * H * pl->bps);
   ok = ok && pl->alloc(reinterpret_cast<void**>(&pl->dBlocks), sizeof(float) * (size_t)pl->nb * bw * bh);
-  if (ok && p.bm == 0)
-    ok = pl->alloc(reinterpret_cast<void**>(&pl->dKalman), sizeof(float4) * (size_t)pl->nb * nbin);
+  if (ok && p.bm == 0) {
+    // FP16 pairs with the FP16 ring on integer samples (PlaneArgs::khalf)
+    pl->kalmanBytes = (pl->half && pl->pix != 0 ? 8 : 16) * (size_t)pl->nb * nbin;
+    ok = pl->alloc(reinterpret_cast<void**>(&pl->dKalman), pl->kalmanBytes);
+  }
   pl->sweepOk = pl->ks.sweep && p.mode == 1 && p.ow * 2 == bw && p.oh * 2 == bh &&
                 sepMatches(pl->t, p);
   if (ok && pl->sweepOk) {
@@ -1472,6 +1476,7 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
   a.nrows = pl->nrows;
   a.nb = pl->nb;
   a.pix = pl->pix;
+  a.khalf = (pl->half && pl->pix != 0) ? 1 : 0;
   a.inNorm = (pl->pix == 16) ? (float)(1u << (16 - desc->bits)) / 65535.0f : 1.0f;
   a.invNorm = (pl->pix == 16) ? 1.0f / (float)(1u << (16 - desc->bits)) : 1.0f;
   a.maxPix = (pl->pix == 16) ? (float)((1u << desc->bits) - 1) : 0.0f;
@@ -1778,6 +1783,21 @@ int fft3d_plane_set_option(fft3d_plane* p, int option, int value) {
     p->pendFn = -1;
     p->half = value == 1;
     p->specBytes = (p->half ? 4 : 8) * (size_t)p->nb * p->p.bh * (p->p.bw / 2);
+    // integer samples keep the Kalman state as FP16 pairs with the FP16 ring
+    // (8 instead of 16 bytes per bin; float samples keep FP32 state, whose
+    // FP16 rounding would exceed the float tolerance over a long clip)
+    p->args.khalf = (p->half && p->pix != 0) ? 1 : 0;
+    if (p->dKalman) {
+      // the state changes format: the recursion starts again at the next frame
+      (void)hipFree(p->dKalman);
+      p->bytes -= p->kalmanBytes;
+      g_bytes -= p->kalmanBytes;
+      p->dKalman = nullptr;
+      p->kalmanBytes = (p->args.khalf ? 8 : 16) * (size_t)p->nb * p->fbw * p->p.bh;
+      p->kalmanFirst = true;
+      if (!p->alloc(reinterpret_cast<void**>(&p->dKalman), p->kalmanBytes))
+        return p->fail(FFT3D_ENOMEM, "GPU alloc failed (Kalman state)");
+    }
     return FFT3D_OK;
   }
   case FFT3D_OPT_RESERVE_SPECTRA:

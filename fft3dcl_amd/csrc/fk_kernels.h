@@ -44,6 +44,7 @@ struct PlaneArgs {
   float sigmaX, sigmaY, betaX, beta;
   float thrX;  // wiener_threshold(sigmaX): the Wiener test is psd > thrX
   float ssmin, ssmax, kalmanCN, kr2;
+  int khalf;  // Kalman state as FP16 pairs (the FP16 ring on integer samples)
   const float* winAna;  // bw*bh analysis window
   const float* winSynA; // bw*bh synthesis window, even block columns (mode 0: winAna)
   const float* winSynB; // bw*bh synthesis window, odd block columns
@@ -346,12 +347,29 @@ __device__ __forceinline__ bool flag(int v) {
   return F == 2 ? v != 0 : F != 0;
 }
 
+// The Kalman state of one bin, {last.re, last.im, covar, proc}: a float4, or
+// with P.khalf (the FP16 ring on integer samples) the same four values as two
+// FP16 pairs rounded to nearest even, 8 bytes instead of 16.
+__device__ __forceinline__ int kstate_bytes(const PlaneArgs& P) { return P.khalf ? 8 : 16; }
+__device__ __forceinline__ float4 kstate_load(const PlaneArgs& P, const void* p) {
+  if (P.khalf) {
+    const uint2 u = *static_cast<const uint2*>(p);
+    const float2 a = unpack_h2(u.x), b = unpack_h2(u.y);
+    return make_float4(a.x, a.y, b.x, b.y);
+  }
+  return *static_cast<const float4*>(p);
+}
+__device__ __forceinline__ void kstate_store(const PlaneArgs& P, void* p, float4 v) {
+  if (P.khalf) *static_cast<uint2*>(p) = make_uint2(pack_h2(v.x, v.y), pack_h2(v.z, v.w));
+  else *static_cast<float4*>(p) = v;
+}
+
 // Temporal filter + sharpen of one bin (src/cpu/reference.cpp:246-409,
 // src/cl/kernels.h:206-342). v[t] are the bin's values in temporal order,
 // d the degrid correction, ks the Kalman state of the bin (bm == 0).
 template <int BM, int DG, int SHP>
 __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4], c32 d,
-                                          float sX, float sY, float lut, float kcn, float4* ks,
+                                          float sX, float sY, float lut, float kcn, void* ks,
                                           bool kfirst) {
   const bool dg = flag<DG>(P.dg), shp = flag<SHP>(P.shp);
   c32 f;
@@ -400,7 +418,7 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
   } else if constexpr (BM == 0) {
     // Kalman (src/cpu/reference.cpp:334-383); first frame initialises state.
     const c32 cu = v[0];
-    float4 st = kfirst ? make_float4(cu.x, cu.y, kcn, kcn) : *ks;
+    float4 st = kfirst ? make_float4(cu.x, cu.y, kcn, kcn) : kstate_load(P, ks);
     const float dr = cu.x - st.x, di = cu.y - st.y;
     const float mask = fmaxf(dr * dr, di * di);
     const float s = st.z + st.w;
@@ -414,7 +432,7 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
       nc = kcn;
       np = kcn;
     }
-    *ks = make_float4(fr, fi, nc, np);
+    kstate_store(P, ks, make_float4(fr, fi, nc, np));
     f = cmake(fr, fi);
   } else {
     f = v[0];  // bm == -1: the current frame passes through
@@ -542,7 +560,8 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   static_assert(NB % D == 0, "whole rounds of batch buffers");
   const size_t base = (size_t)b * S::BSTR;
   const int c = lb % NCOL, r0 = lb / NCOL;
-  float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
+  const int kst = kstate_bytes(P);
+  char* kb = kstate ? reinterpret_cast<char*>(kstate) + (size_t)b * BH * FBW * kst : nullptr;
   float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if constexpr (SEP) cf = col_factor(tab, BH, c, dc);
   // All four slots are loaded and staged: slots past nwin repeat a window
@@ -571,7 +590,7 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
         if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP, PAT>(P, r, c, tab[r], cf);
         else kc = bin_const<BW, DG, SHP, PAT>(P, r, c, dc);
         Yb[S::yidx(c, r)] = filter_bin<BM, DG, SHP>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
-                                       kb ? kb + r * FBW + c : nullptr, kfirst);
+                                       kb ? kb + (size_t)(r * FBW + c) * kst : nullptr, kfirst);
       }
     }
   };
@@ -603,7 +622,8 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
   using S = Shape<BW, BH>;
   constexpr int FBW = S::FBW, NCOL = S::NCOL, H2 = BH / 2;
   const int nwin = Wn.nwin;
-  float4* kb = kstate ? kstate + (size_t)b * BH * FBW : nullptr;
+  const int kst = kstate_bytes(P);
+  char* kb = kstate ? reinterpret_cast<char*>(kstate) + (size_t)b * BH * FBW * kst : nullptr;
   float4 cf0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cfh = cf0;
   if constexpr (SEP) {
     cf0 = col_factor(tab, BH, 0, dc);
@@ -631,7 +651,7 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
       if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP, PAT>(P, r, col, tab[r], col == 0 ? cf0 : cfh);
       else kc = bin_const<BW, DG, SHP, PAT>(P, r, col, dc);
       const c32 y = filter_bin<BM, DG, SHP>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
-                               kb ? kb + r * FBW + col : nullptr, kfirst);
+                               kb ? kb + (size_t)(r * FBW + col) * kst : nullptr, kfirst);
       // f[k] with a runtime k would live in scratch: select instead
       f[0] = k == 0 ? y : f[0];
       f[1] = k == 1 ? y : f[1];

@@ -254,7 +254,11 @@ int fft3d_plane_get_profile(fft3d_plane* p, fft3d_kernel_time* out, int max);
  *                        on gfx950 so far: the sweep is latency-bound).
  *   FFT3D_OPT_SPECTRA_FP16  1: stores the spectra ring as FP16 pairs (round
  *                        to nearest even; arithmetic stays FP32), half the
- *                        ring's bytes; drops the ring. Square blocks of 16,
+ *                        ring's bytes; drops the ring. With bt=0 on integer
+ *                        samples the Kalman state (last value, covariance,
+ *                        process noise) is stored as FP16 too, 8 instead of
+ *                        16 bytes per bin; the recursion starts again at the
+ *                        next frame. Square blocks of 16,
  *                        32 and 64 only (FFT3D_EUNSUPPORTED otherwise). The
  *                        reference's precision=0 storage; not exact.
  *   FFT3D_OPT_RESERVE_SPECTRA  allocates that many spectra into the reuse

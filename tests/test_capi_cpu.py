@@ -522,6 +522,55 @@ def test_emu_fp16_ring_within_parity(oracle, emu_lib, kind, kw):
         assert lsb_diff(got[0], want[0]) <= (2 if bits == 8 else 4)
 
 
+@pytest.mark.parametrize("kind,kw", [("u8", dict(bt=0, bw=16, bh=16)),
+                                     ("u10", dict(bt=0, bw=32, bh=32, kratio=1.5)),
+                                     ("u8", dict(bt=0, bw=64, bh=64, sigma2=3.0, sigma3=2.0,
+                                                 sigma4=1.0))])
+def test_emu_fp16_kalman_state_within_parity(oracle, emu_lib, kind, kw):
+    """With the FP16 ring on integer samples the Kalman state is stored as
+    FP16 too: every output frame of a 16-frame clip with a scene cut at frame
+    8 (the kratio reset) stays within the path's parity tolerance of the
+    oracle, whose state is FP32. (Float samples keep FP32 state; the FP16 ring
+    alone already reaches 1.9e-3 of range on such a float clip, above the
+    float tolerance, so the FP16 ring is a form for integer clips.)"""
+    bits, st = kind_info(kind)
+    planes = make_clip(29, 96, 80, 16, kind, scene_cut=8)
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st)
+    got = FFT3DCL(planes, bits=bits, sample_type=st, lib=emu_lib, half=True, **kw)
+    for n in range(16):
+        if kind == "f32":
+            assert max_abs(got[0][n], want[0][n]) < 1e-3, n
+        else:
+            assert lsb_diff(got[0][n], want[0][n]) <= (2 if bits == 8 else 4), n
+
+
+def test_emu_fp16_option_restarts_kalman(emu_lib):
+    """The Kalman state of integer samples follows the ring: FP16 pairs (8
+    bytes per bin) with the FP16 ring, as precision 0 gives 8-bit clips at
+    creation, FP32 (16 bytes) otherwise; switching reallocates it. Float
+    samples keep FP32 state."""
+    pl = capi.Plane(capi.make_params(bt=0, precision=2), 96, 72, 10, lib=emu_lib)
+    nbin = pl.info.nb * pl.info.fbw * pl.info.bh
+    before = capi.live_bytes(emu_lib)
+    pl.set_option(capi.Plane.OPT_SPECTRA_FP16, 1)
+    assert before - capi.live_bytes(emu_lib) == 8 * nbin
+    pl.set_option(capi.Plane.OPT_SPECTRA_FP16, 0)
+    assert capi.live_bytes(emu_lib) == before
+    pl.close()
+    before = capi.live_bytes(emu_lib)
+    p8 = capi.Plane(capi.make_params(bt=0), 96, 72, 8, lib=emu_lib)  # precision 0: FP16
+    p8.set_option(capi.Plane.OPT_SPECTRA_FP16, 0)
+    grown = capi.live_bytes(emu_lib)
+    p8.set_option(capi.Plane.OPT_SPECTRA_FP16, 1)
+    assert grown - capi.live_bytes(emu_lib) == 8 * nbin
+    p8.close()
+    pf = capi.Plane(capi.make_params(bt=0), 96, 72, 32, sample_type=1, lib=emu_lib)
+    before = capi.live_bytes(emu_lib)
+    pf.set_option(capi.Plane.OPT_SPECTRA_FP16, 1)
+    assert capi.live_bytes(emu_lib) == before
+    pf.close()
+
+
 def test_emu_fp16_ring_stores_rounded_spectra(emu_lib):
     frames = make_clip(18, 96, 72, 1, "f32")[0]
     full = capi.Plane(capi.make_params(), 96, 72, 32, sample_type=1, lib=emu_lib)

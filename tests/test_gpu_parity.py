@@ -98,6 +98,34 @@ def test_config_d_full_kalman_run(gpu, oracle):
     assert max(d) <= tolerance("f32"), d
 
 
+@pytest.mark.parametrize("kind,kw,half", [
+    ("u8", dict(bt=0, bw=16, bh=16, sigma=2.5), False),       # precision 0: FP16 ring
+    ("u10", dict(bt=0, bw=64, bh=64, sigma=3.0, sigma2=4.0, sigma3=2.0, sigma4=1.0), True)])
+def test_fp16_kalman_long_run(gpu, oracle, kind, kw, half):
+    """The FP16 ring on integer samples with Kalman (bt=0): the state is FP16
+    pairs too. 1280x720, 48 frames with a scene cut at frame 24: every frame
+    within the path's tolerance of the oracle (FP32 state), device memory
+    flat over the clip."""
+    W, H, n = 1280, 720, 48
+    bits, st = kind_info(kind)
+    planes = make_clip(1050 + bits, W, H, n, kind, scene_cut=24)
+    want, _ = oracle.run_clip(script(**kw), planes, bits, st)
+    inst = Instance(1, W, H, bits, st, n, lib=gpu, half=half, **kw)
+    src = torch.from_numpy(planes[0]).cuda()
+    out = torch.zeros_like(src)
+    dev = []
+    for k in range(n):
+        inst.get_frame(k, [lambda fn: src[fn]], [out[k]], on_device=True)
+        dev.append(inst.objs[0].info.device_bytes)
+    torch.cuda.synchronize()
+    inst.close()
+    assert max(dev[1:]) == min(dev[1:]), dev
+    got = out.cpu().numpy()
+    d = [lsb_diff(got[k], want[0][k]) for k in range(n)]
+    print(f"FP16 Kalman {kind} bw{kw['bw']}: max {max(d)} LSB (frame {int(np.argmax(d))})")
+    assert max(d) <= tolerance(kind), d
+
+
 EDGE = [
     ("u8", 64, 48, 1, dict(bt=3)),            # one-frame clip: window clamps to [0,0,0]
     ("u8", 64, 48, 2, dict(bt=4)),            # two frames, bt 4
