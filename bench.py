@@ -23,7 +23,11 @@ else the oracle restatement) single-threaded, in steady state, on configs
 A-D (value: the 4K workload).
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-host] [--no-served]
-                    [--no-variants]
+                    [--no-variants] [--form NAME]
+
+--form runs the main loop in another precision form of the same workload
+(a name from FORMS; default the plain FP32 form, which `value` is quoted
+on); the result line then names the form in `config.workload`.
 """
 import argparse
 import json
@@ -43,11 +47,12 @@ SEED = 1002
 NUNIQ = 16  # distinct synthetic frames, cycled through the clip
 
 
-def kernel_bytes(info, bps_in, bps_out):
+def kernel_bytes(info, bps_in, bps_out, bin_bytes=8):
     """Algorithmic HBM bytes per launch of each kernel. A spectrum is
-    bins * 8 B with bins = nb * (bw/2+1) * bh (complex FP32)."""
+    bins * bin_bytes with bins = nb * (bw/2+1) * bh (8: complex FP32; 4: the
+    FP16 ring)."""
     bins = info.nb * info.fbw * info.bh
-    spec = bins * 8
+    spec = bins * bin_bytes
     return {
         "analysis": W * H * bps_in + spec,          # read the plane, write one spectrum
         "filter_inverse": spec * info.bt,           # read the bt-frame window
@@ -336,15 +341,15 @@ def prefetch_variant(dev, steps=100, warmup=10):
     return res
 
 
-def served_sample(seconds=0.5, nstreams=32, uniq=3):
-    """The served form (32 seeded streams of mixed resolution, bit depth, bt
-    and block size, Poisson arrivals at 24-60 frames/s each over `seconds`):
-    p50/p99 latency at that offered load, and the capacity of the same
-    requests issued all at once, one call per request and batched through
-    fft3d_step. Frames are seeded noise made on the GPU (`uniq` per stream,
-    cycled): the filter's cost does not depend on them."""
+SERVED_LOOPS = {"one_queue": dict(one_queue=True), "streams": {},
+                "batched": dict(batched=True),
+                "batched_fp16_mfma": dict(batched=True, form="fp16_mfma")}
+
+
+def served_setup(seconds=0.5, nstreams=32, uniq=3):
+    """The served section's streams, their seeded device frames and outputs."""
     import torch
-    from fft3dcl_amd.serve import Server, make_specs, summarize
+    from fft3dcl_amd.serve import make_specs
     specs = make_specs(SEED, nstreams, duration=seconds)
     g = torch.Generator(device="cuda").manual_seed(SEED)
     src, out = {}, {}
@@ -354,6 +359,33 @@ def served_sample(seconds=0.5, nstreams=32, uniq=3):
                                     device="cuda", dtype=torch.int32).to(dt)
         out[sp.sid] = torch.empty((sp.height, sp.width), dtype=dt, device="cuda")
     sources = {sid: (lambda fn, t=t: t[fn % uniq]) for sid, t in src.items()}
+    return specs, sources, out
+
+
+def served_pass(loop):
+    """One capacity pass (every request issued at once) of one served loop
+    after a warm-up pass of it: the unit that `rocprofv3 --kernel-trace`
+    profiles. One JSON line."""
+    from fft3dcl_amd.serve import Server, summarize
+    specs, sources, out = served_setup()
+    def run():
+        return Server(specs, sources, lambda sid, n: out[sid], 64 << 30,
+                      **SERVED_LOOPS[loop]).run(paced=False)
+    run()
+    res = summarize(specs, run())
+    print(json.dumps({"served_pass": loop, "frames_per_s": res["frames_per_s"],
+                      "requests": res["requests"]}), flush=True)
+
+
+def served_sample(seconds=0.5, nstreams=32, uniq=3):
+    """The served form (32 seeded streams of mixed resolution, bit depth, bt
+    and block size, Poisson arrivals at 24-60 frames/s each over `seconds`):
+    p50/p99 latency at that offered load, and the capacity of the same
+    requests issued all at once, one call per request and batched through
+    fft3d_step. Frames are seeded noise made on the GPU (`uniq` per stream,
+    cycled): the filter's cost does not depend on them."""
+    from fft3dcl_amd.serve import Server, summarize
+    specs, sources, out = served_setup(seconds, nstreams, uniq)
     budget = 64 << 30
     Server(specs, sources, lambda sid, n: out[sid], budget).run(paced=False)  # warm-up
     def run(paced, **kw):
@@ -410,6 +442,10 @@ def main():
                     help="skip the precision forms and the device-prefetch variant")
     ap.add_argument("--host-only", action="store_true",
                     help="only measure the host-buffer rates (one JSON line of them)")
+    ap.add_argument("--form", default=FORMS[0][0], choices=[f[0] for f in FORMS],
+                    help="precision form of the main loop (default: the plain FP32 form)")
+    ap.add_argument("--served-pass", choices=sorted(SERVED_LOOPS), default=None,
+                    help="only one capacity pass of that served loop (for kernel traces)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB, --steps times (the "
                          "reference point for HBM byte counters)")
@@ -439,6 +475,9 @@ def main():
         return
 
     capi.dev_init(0)
+    if args.served_pass:
+        served_pass(args.served_pass)
+        return
     frames = make_plane_sequence(SEED, W, H, NUNIQ, "u10")
     if args.host_only:
         print(json.dumps({"host_buffers_pcie_inclusive": host_buffer_rates(frames)}), flush=True)
@@ -446,7 +485,8 @@ def main():
     dev = torch.from_numpy(frames).to("cuda")
     out = torch.empty((H, W), dtype=torch.int16, device="cuda")
     nframes = max(120, args.warmup + 2 * args.steps + 8)
-    inst = Instance(GRAY, W, H, BITS, 0, nframes, **PARAMS)
+    form_kw = {f[0]: f[1] for f in FORMS}[args.form]
+    inst = Instance(GRAY, W, H, BITS, 0, nframes, **form_kw, **PARAMS)
     plane = inst.objs[0]
     info = plane.info
     stream = torch.cuda.Stream()
@@ -484,7 +524,7 @@ def main():
 
     ms_step = ms_total / args.steps
     fps = 1000.0 / ms_step
-    kb = kernel_bytes(info, 2, 2)
+    kb = kernel_bytes(info, 2, 2, 4 if form_kw.get("half") else 8)
     kernels = {}
     for name, (launches, tot) in prof.items():
         avg = tot / launches
@@ -495,7 +535,10 @@ def main():
     traffic = None
     tf = ROOT / "profiles" / "traffic.json"
     if tf.exists():
-        t = json.loads(tf.read_text()).get(dom)
+        tj = json.loads(tf.read_text())
+        if args.form != FORMS[0][0]:
+            tj = tj.get("forms", {}).get(args.form, {})
+        t = tj.get(dom)
         traffic = t.get("bytes_per_launch") if isinstance(t, dict) else None
     ach = kernels[dom]["GBps"]
     step_gbs = kb["step"] / (ms_step * 1e-3) / 1e9
@@ -505,7 +548,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "4K 10-bit luma 3840x2160, sigma=3 bt=4 bw=bh=32 ow=oh=16 "
-                               "sharpen=0.4 degrid=1, FP32 spectra ring",
+                               "sharpen=0.4 degrid=1, " +
+                               ("FP32 spectra ring" if args.form == FORMS[0][0]
+                                else f"form {args.form}"),
                    "frames": nframes, "distinct_frames": NUNIQ, "seed": SEED,
                    "mpix_per_s": round(fps * W * H / 1e6, 1),
                    "step_algorithmic_bytes": kb["step"],
