@@ -165,7 +165,7 @@ def gpu_configs(steps=40, warmup=6):
     return res
 
 
-def host_buffer_rates(frames, steps=24):
+def host_buffer_rates(frames, steps=60, repeats=3):
     """Frames/s of the same workload when the planes enter and leave through
     host buffers (the VapourSynth plugin's boundary): PCIe included, both
     directions, as the C ABI does it (host-to-device copy before the analysis,
@@ -173,7 +173,10 @@ def host_buffer_rates(frames, steps=24):
     side is done). Pageable numpy buffers through the library's pinned chunks
     (its default), the same buffers handed to hipMemcpyAsync as they are, and
     pinned buffers; then pageable and pinned buffers with each request
-    prefetching the next one's new frame (its upload overlaps the request)."""
+    prefetching the next one's new frame (its upload overlaps the request).
+    Each rate is the median of `repeats` runs of `steps` frames (the spread,
+    min and max, is reported beside it: host copies share the CPU with
+    everything else on the machine)."""
     import numpy as np
     import torch
     from fft3dcl_amd.filter import GRAY, Instance
@@ -182,7 +185,8 @@ def host_buffer_rates(frames, steps=24):
     out_pinned = torch.empty((H, W), dtype=torch.int16).pin_memory()
     for kind in ("pageable", "pageable_unstaged", "pinned", "pageable_prefetch",
                  "pinned_prefetch"):
-        inst = Instance(GRAY, W, H, BITS, 0, steps + 16, prefetch=kind.endswith("prefetch"),
+        inst = Instance(GRAY, W, H, BITS, 0, repeats * steps + 16,
+                        prefetch=kind.endswith("prefetch"),
                         **PARAMS)
         if kind == "pageable_unstaged":
             inst.objs[0].set_option(inst.objs[0].OPT_HOST_STAGING, 0)
@@ -195,13 +199,19 @@ def host_buffer_rates(frames, steps=24):
             dst = out_pinned
         for n in range(4):
             inst.get_frame(n, srcs, [dst])
-        t0 = time.perf_counter()
-        for n in range(4, 4 + steps):
-            inst.get_frame(n, srcs, [dst])
-        dt = time.perf_counter() - t0
+        rates = []
+        for rep in range(repeats):
+            base = 4 + rep * steps
+            t0 = time.perf_counter()
+            for n in range(base, base + steps):
+                inst.get_frame(n, srcs, [dst])
+            rates.append(steps / (time.perf_counter() - t0))
         inst.close()
-        res[kind + "_fps"] = round(steps / dt, 1)
+        rates.sort()
+        res[kind + "_fps"] = round(rates[len(rates) // 2], 1)
+        res[kind + "_fps_range"] = [round(rates[0], 1), round(rates[-1], 1)]
     res["frames"] = steps
+    res["repeats"] = repeats
     return res
 
 
