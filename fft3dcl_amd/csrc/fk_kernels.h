@@ -348,26 +348,28 @@ __device__ __forceinline__ bool flag(int v) {
 }
 
 // The Kalman state of one bin, {last.re, last.im, covar, proc}: a float4, or
-// with P.khalf (the FP16 ring on integer samples) the same four values as two
-// FP16 pairs rounded to nearest even, 8 bytes instead of 16.
-__device__ __forceinline__ int kstate_bytes(const PlaneArgs& P) { return P.khalf ? 8 : 16; }
-__device__ __forceinline__ float4 kstate_load(const PlaneArgs& P, const void* p) {
-  if (P.khalf) {
+// with KH (P.khalf: the FP16 ring on integer samples) the same four values as
+// two FP16 pairs rounded to nearest even, 8 bytes instead of 16.
+template <bool KH>
+__device__ __forceinline__ float4 kstate_load(const void* p) {
+  if constexpr (KH) {
     const uint2 u = *static_cast<const uint2*>(p);
     const float2 a = unpack_h2(u.x), b = unpack_h2(u.y);
     return make_float4(a.x, a.y, b.x, b.y);
+  } else {
+    return *static_cast<const float4*>(p);
   }
-  return *static_cast<const float4*>(p);
 }
-__device__ __forceinline__ void kstate_store(const PlaneArgs& P, void* p, float4 v) {
-  if (P.khalf) *static_cast<uint2*>(p) = make_uint2(pack_h2(v.x, v.y), pack_h2(v.z, v.w));
+template <bool KH>
+__device__ __forceinline__ void kstate_store(void* p, float4 v) {
+  if constexpr (KH) *static_cast<uint2*>(p) = make_uint2(pack_h2(v.x, v.y), pack_h2(v.z, v.w));
   else *static_cast<float4*>(p) = v;
 }
 
 // Temporal filter + sharpen of one bin (src/cpu/reference.cpp:246-409,
 // src/cl/kernels.h:206-342). v[t] are the bin's values in temporal order,
 // d the degrid correction, ks the Kalman state of the bin (bm == 0).
-template <int BM, int DG, int SHP>
+template <int BM, int DG, int SHP, bool KH = false>
 __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4], c32 d,
                                           float sX, float sY, float lut, float kcn, void* ks,
                                           bool kfirst) {
@@ -418,7 +420,7 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
   } else if constexpr (BM == 0) {
     // Kalman (src/cpu/reference.cpp:334-383); first frame initialises state.
     const c32 cu = v[0];
-    float4 st = kfirst ? make_float4(cu.x, cu.y, kcn, kcn) : kstate_load(P, ks);
+    float4 st = kfirst ? make_float4(cu.x, cu.y, kcn, kcn) : kstate_load<KH>(ks);
     const float dr = cu.x - st.x, di = cu.y - st.y;
     const float mask = fmaxf(dr * dr, di * di);
     const float s = st.z + st.w;
@@ -432,7 +434,7 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
       nc = kcn;
       np = kcn;
     }
-    kstate_store(P, ks, make_float4(fr, fi, nc, np));
+    kstate_store<KH>(ks, make_float4(fr, fi, nc, np));
     f = cmake(fr, fi);
   } else {
     f = v[0];  // bm == -1: the current frame passes through
@@ -545,7 +547,7 @@ __device__ __forceinline__ c32 spec_value(typename SpecWord<H>::T v) {
 // prefetch does not pay: the sweep is bound by its compute phases.
 constexpr int kPass1Rows = 4, kPass1Buffers = 2;
 
-template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT>
+template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT, bool KH = false>
 __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
@@ -560,7 +562,7 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   static_assert(NB % D == 0, "whole rounds of batch buffers");
   const size_t base = (size_t)b * S::BSTR;
   const int c = lb % NCOL, r0 = lb / NCOL;
-  const int kst = kstate_bytes(P);
+  constexpr int kst = KH ? 8 : 16;  // bytes of a bin's Kalman state
   char* kb = kstate ? reinterpret_cast<char*>(kstate) + (size_t)b * BH * FBW * kst : nullptr;
   float4 cf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if constexpr (SEP) cf = col_factor(tab, BH, c, dc);
@@ -589,7 +591,7 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
         BinConst kc;
         if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP, PAT>(P, r, c, tab[r], cf);
         else kc = bin_const<BW, DG, SHP, PAT>(P, r, c, dc);
-        Yb[S::yidx(c, r)] = filter_bin<BM, DG, SHP>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
+        Yb[S::yidx(c, r)] = filter_bin<BM, DG, SHP, KH>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                        kb ? kb + (size_t)(r * FBW + c) * kst : nullptr, kfirst);
       }
     }
@@ -615,14 +617,14 @@ __device__ __forceinline__ void filter_pass1(const PlaneArgs& P, const WindowArg
   }
 }
 
-template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT>
+template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT, bool KH = false>
 __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                              int lb, c32* Yb, const float4* tab, float4* kstate,
                                              bool kfirst, float dc) {
   using S = Shape<BW, BH>;
   constexpr int FBW = S::FBW, NCOL = S::NCOL, H2 = BH / 2;
   const int nwin = Wn.nwin;
-  const int kst = kstate_bytes(P);
+  constexpr int kst = KH ? 8 : 16;  // bytes of a bin's Kalman state
   char* kb = kstate ? reinterpret_cast<char*>(kstate) + (size_t)b * BH * FBW * kst : nullptr;
   float4 cf0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), cfh = cf0;
   if constexpr (SEP) {
@@ -650,7 +652,7 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
       BinConst kc;
       if constexpr (SEP) kc = bin_const_sep<BW, DG, SHP, PAT>(P, r, col, tab[r], col == 0 ? cf0 : cfh);
       else kc = bin_const<BW, DG, SHP, PAT>(P, r, col, dc);
-      const c32 y = filter_bin<BM, DG, SHP>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
+      const c32 y = filter_bin<BM, DG, SHP, KH>(P, v, kc.d, kc.sX, kc.sY, kc.lut, kc.kcn,
                                kb ? kb + (size_t)(r * FBW + col) * kst : nullptr, kfirst);
       // f[k] with a runtime k would live in scratch: select instead
       f[0] = k == 0 ? y : f[0];
@@ -671,15 +673,15 @@ __device__ __forceinline__ void filter_pass2(const PlaneArgs& P, const WindowArg
   }
 }
 
-template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT>
+template <int BW, int BH, bool SEP, bool H, int BM, int DG, int SHP, int PAT, bool KH = false>
 __device__ __forceinline__ void filter_passes(const PlaneArgs& P, const WindowArgs& Wn, int b,
                                               bool live, int lb, c32* Yb, const float4* tab,
                                               float4* kstate, bool kfirst, float dc) {
   if (live)
-    filter_pass1<BW, BH, SEP, H, BM, DG, SHP, PAT>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+    filter_pass1<BW, BH, SEP, H, BM, DG, SHP, PAT, KH>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
   wave_sync();
   if (live)
-    filter_pass2<BW, BH, SEP, H, BM, DG, SHP, PAT>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
+    filter_pass2<BW, BH, SEP, H, BM, DG, SHP, PAT, KH>(P, Wn, b, lb, Yb, tab, kstate, kfirst, dc);
 }
 
 // The Wiener modes get one instance per (degrid, sharpen, sigma pattern)
@@ -702,6 +704,13 @@ __device__ __forceinline__ void filter_mode(const PlaneArgs& P, const WindowArgs
     case 1: FK_PASSES(0, 0, 1); break;
     default: FK_PASSES(0, 0, 0);
     }
+  } else if constexpr (BM == 0 && H) {
+    // Kalman over an FP16 ring: the state is FP16 pairs on integer samples
+    if (P.khalf)
+      filter_passes<BW, BH, SEP, H, 0, 2, 2, 2, true>(P, Wn, b, live, lb, Yb, tab, kstate, kfirst,
+                                                       dc);
+    else
+      FK_PASSES(2, 2, 2);
   } else {
     FK_PASSES(2, 2, 2);
   }
