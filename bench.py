@@ -368,7 +368,10 @@ def served_setup(seconds=0.5, nstreams=32, uniq=3):
         src[sp.sid] = torch.randint(0, 1 << sp.bits, (uniq, sp.height, sp.width), generator=g,
                                     device="cuda", dtype=torch.int32).to(dt)
         out[sp.sid] = torch.empty((sp.height, sp.width), dtype=dt, device="cuda")
-    sources = {sid: (lambda fn, t=t: t[fn % uniq]) for sid, t in src.items()}
+    # each stream's frames as ready device views (a server holds its frame
+    # buffers; the callback only picks one)
+    views = {sid: [t[i] for i in range(uniq)] for sid, t in src.items()}
+    sources = {sid: (lambda fn, v=v: v[fn % uniq]) for sid, v in views.items()}
     return specs, sources, out
 
 

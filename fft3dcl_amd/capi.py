@@ -356,23 +356,26 @@ def step(requests: list, lib=None) -> list[int]:
     batched host step) and returns each item's status (0 = FFT3D_OK). The
     buffers the sources hand out are kept alive until the call returns."""
     L = load(lib)
-    keep, cbs = [], []
+    keep = []
+
+    def cb(user, fn):
+        # one callback for every item; `user` is the item's index + 1
+        r = requests[user - 1]
+        buf = r.source(fn)
+        if buf is None:
+            return None
+        try:
+            ptr, _ = r.plane.buffer(buf, r.on_device, "step source")
+        except ValueError:
+            return None  # the item fails with "source gave no plane"
+        keep.append(buf)
+        return ptr
+
+    c = SOURCE_FN(cb)
     items = (WorkItem * max(1, len(requests)))()
     for k, r in enumerate(requests):
-        def cb(_user, fn, src=r.source, pl=r.plane, dev=r.on_device):
-            buf = src(fn)
-            if buf is None:
-                return None
-            try:
-                ptr, _ = pl.buffer(buf, dev, "step source")
-            except ValueError:
-                return None  # the item fails with "source gave no plane"
-            keep.append(buf)
-            return ptr
-        c = SOURCE_FN(cb)
-        cbs.append(c)
         dptr, dn = r.plane.buffer(r.dst, r.on_device, "step output")
-        items[k] = WorkItem(r.plane.h, r.frame, r.num_frames, c, None, int(r.on_device),
+        items[k] = WorkItem(r.plane.h, r.frame, r.num_frames, c, k + 1, int(r.on_device),
                             dptr, dn, int(r.on_device), r.stream, 0)
     L.fft3d_step(items, len(requests))
     return [items[k].status for k in range(len(requests))]

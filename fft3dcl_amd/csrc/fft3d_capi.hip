@@ -1270,8 +1270,15 @@ void step_batch(fft3d_work_item* items, const std::vector<std::vector<int>>& rou
       (void)hipGetLastError();
       return fail_all(FFT3D_ELAUNCH, "step stream ordering failed");
     }
-  for (int k : all_items)
-    if (items[k].status == FFT3D_OK) items[k].plane->end_call(bs);
+  // one end-of-call event per plane (a plane has one item per round)
+  std::vector<fft3d_plane*> ended;
+  for (int k : all_items) {
+    fft3d_plane* p = items[k].plane;
+    if (items[k].status != FFT3D_OK || std::find(ended.begin(), ended.end(), p) != ended.end())
+      continue;
+    ended.push_back(p);
+    p->end_call(bs);
+  }
 }
 
 }  // namespace
