@@ -212,6 +212,7 @@ struct fft3d_plane {
   fft3d_plane_desc d{};
   int nb = 0, ncols = 0, nrows = 0, fbw = 0, bps = 4, pix = 0;
   int ncu = 0;  // compute units of the plane's device
+  int mfmaPerCU = 0;  // resident MFMA analysis workgroups per CU (0: unknown)
   size_t specBytes = 0;
   KernelSet ks;
   fk::PlaneArgs args{};
@@ -650,12 +651,15 @@ struct fft3d_plane {
   }
 
   // Analysis of `plane` into `spec` on stream s (register FFTs, or the MFMA
-  // form: 32 blocks per 256-thread workgroup).
+  // form: 256-thread workgroups of at least 32 blocks, at most one round of
+  // them on the device).
   void launch_analysis(const void* plane, fk::c32* spec, hipStream_t s) {
     hipEvent_t t0 = prof_begin(s);
     if (dft) {
       AnalysisFn k = ks.analysisM[dft - 1][half ? 1 : 0];
-      FK_LAUNCH(k, dim3((nb + 31) / 32), dim3(256), s, args, plane, spec);
+      int grid = (nb + 31) / 32;
+      if (mfmaPerCU > 0 && ncu > 0) grid = std::min(grid, mfmaPerCU * ncu);
+      FK_LAUNCH(k, dim3(grid), dim3(256), s, args, plane, spec);
     } else {
       const int grid = (nb + ks.G - 1) / ks.G;
       AnalysisFn k = half ? ks.analysisH : ks.analysis;
@@ -1375,6 +1379,12 @@ int fft3d_plane_create(const fft3d_params* sp, const fft3d_plane_desc* desc, fft
     if (hipGetDevice(&dev) == hipSuccess) pl->dev = dev;
     if (hipGetDeviceProperties(&prop, pl->dev) == hipSuccess)
       pl->ncu = prop.multiProcessorCount;
+    // the MFMA analysis instances share one resource footprint
+    int n = 0;
+    if (pl->ks.analysisM[0][0] &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void*>(pl->ks.analysisM[0][0]), 256, 0) == hipSuccess)
+      pl->mfmaPerCU = n;
     (void)hipGetLastError();
   }
   if (!pl->ks.analysis) {

@@ -380,10 +380,12 @@ __device__ __forceinline__ void inverse_block_mfma(const InvOps& O, c32* Yb, int
 
 }  // namespace mm
 
-// Analysis by MFMA DFT: 4 waves per workgroup, 8 blocks per wave; the
-// workgroup's 32 consecutive blocks (XCD-contiguous, see xcd_contiguous) go
-// 4 at a time to its waves, so neighbouring blocks, which share pixel rows,
-// are read together.
+// Analysis by MFMA DFT: 4 waves per workgroup. The plane's blocks form
+// chunks of 4 consecutive blocks, one block per wave; workgroup wg of nwg
+// takes a contiguous run of nch / nwg chunks (balanced, XCD-contiguous, see
+// xcd_contiguous), so neighbouring blocks, which share pixel rows, are read
+// together. A grid of one round of resident workgroups (each with a run of
+// many chunks) leaves no ragged last round.
 // The forward constant operands go to LDS once per workgroup; each wave
 // loads the samples of its next block while it transforms the current one.
 template <int PR, bool H>
@@ -392,15 +394,17 @@ __device__ __forceinline__ void analysis_mfma_body(const PlaneArgs& P, const voi
   __shared__ uint4 ctab[12 * 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int i = tid; i < 12 * 64; i += 256) ctab[i] = P.dft[i];
-  const int g = xcd_contiguous(wg, nwg);
+  const int nch = (P.nb + 3) / 4;
+  const long long j = xcd_contiguous(wg, nwg);
+  const int c0 = (int)(j * nch / nwg), c1 = (int)((j + 1) * nch / nwg);
   float wf[16], xs[16], xn[16];
   mm::window32(P, lane, wf);
-  int b = g * 32 + wv;
-  if (b < P.nb) mm::load_block32(P, plane, b, lane, wf, xs);
+  int b = c0 * 4 + wv;
+  if (c0 < c1 && b < P.nb) mm::load_block32(P, plane, b, lane, wf, xs);
   __syncthreads();  // the constant operands are in LDS
-  for (int i = 0; i < 8 && b < P.nb; i++) {  // b is wave-uniform
+  for (int c = c0; c < c1 && b < P.nb; c++) {  // b is wave-uniform
     const int bn = b + 4;
-    if (i + 1 < 8 && bn < P.nb) mm::load_block32(P, plane, bn, lane, wf, xn);
+    if (c + 1 < c1 && bn < P.nb) mm::load_block32(P, plane, bn, lane, wf, xn);
     mm::forward_block_mfma<PR, H>(xs, ctab, b, spec, lane);
 #pragma unroll
     for (int k = 0; k < 16; k++) xs[k] = xn[k];

@@ -207,7 +207,15 @@ inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
   std::strcpy(p->gcnArchName, "emu");
   p->sharedMemPerBlock = 160 * 1024;
   p->totalGlobalMem = (size_t)1 << 34;
-  p->multiProcessorCount = 256;  // the emulated part: MI355X
+  // the emulated part: MI355X (FFT3D_EMU_CUS sets fewer, so that small
+  // planes take the paths of planes larger than one round of workgroups)
+  const char* cus = std::getenv("FFT3D_EMU_CUS");
+  p->multiProcessorCount = cus && std::atoi(cus) > 0 ? std::atoi(cus) : 256;
+  return hipSuccess;
+}
+// two resident workgroups per CU, whatever the kernel
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) {
+  *n = 2;
   return hipSuccess;
 }
 inline hipError_t hipGetLastError() { return hipSuccess; }

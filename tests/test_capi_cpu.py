@@ -678,6 +678,24 @@ def test_emu_mfma_form_option_errors(emu_lib):
     pl.close()
 
 
+def test_emu_mfma_analysis_runs(emu_lib, monkeypatch):
+    """The MFMA analysis on a grid of fewer workgroups than 32-block groups
+    (each workgroup then takes a longer run of 4-block chunks, the one-round
+    grid of a large plane) gives the spectra of the default grid, bit for
+    bit, ragged last chunk included."""
+    W, H = 160, 128  # 11 x 9 = 99 blocks of 32x32: 4 groups of 32
+    frame = np.ascontiguousarray(make_clip(7, W, H, 1, "u8")[0][0])
+    specs = []
+    for cus in ("256", "1"):  # one emulated CU holds 2 workgroups
+        monkeypatch.setenv("FFT3D_EMU_CUS", cus)
+        pl = capi.Plane(capi.make_params(bt=3), W, H, 8, lib=emu_lib)
+        pl.set_option(capi.Plane.OPT_DFT, 1)
+        pl.forward(0, frame)
+        specs.append(pl.read_spectrum(0))
+        pl.close()
+    assert np.array_equal(specs[0], specs[1])
+
+
 def _raw_step(lib, reqs):
     """fft3d_step over (plane, frame, nframes, clip, out, on_device) with
     numpy buffers: in the emulation, device memory is host memory, so the
