@@ -334,9 +334,14 @@ __host__ __device__ inline float wiener_threshold(float sX) {
   // 1/(+0) = +inf passes every psd; 1/(-0) and 1/negative never pass.
   return (sX == 0.0f && copysignf(1.0f, sX) > 0.0f) ? -1.0f : __builtin_inff();
 }
-__device__ __forceinline__ c32 wiener(c32 v, float thr, float sY, float betaX) {
-  const float psd = v.x * v.x + v.y * v.y + 1e-37f;
-  const float mul = psd > thr ? 1.0f - sY * frcp(psd) : betaX;
+// With `one` a power of two (the mean's 1/2 or 1/4 folded in, sY and betaX
+// scaled by it) the result is exactly `one` times the unscaled gain's. The
+// 1e-37 goes in with y*y: it only decides psd > 0 for a zero bin, and a bin
+// whose psd it could change takes the betaX branch either way.
+__device__ __forceinline__ c32 wiener(c32 v, float thr, float sY, float betaX,
+                                      float one = 1.0f) {
+  const float psd = fmaf(v.x, v.x, fmaf(v.y, v.y, 1e-37f));
+  const float mul = psd > thr ? one - sY * frcp(psd) : betaX;
   return cscale(v, mul);
 }
 
@@ -379,9 +384,11 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
     const c32 pv = v[0], cu = v[1];
     c32 a = csub(cu, pv), bb = cadd(cu, pv);
     if (dg) bb = csub(bb, cscale(d, 2.0f));
-    a = wiener(a, sX, sY, P.betaX);
-    bb = wiener(bb, sX, sY, P.betaX);
-    f = cscale(cadd(a, bb), 0.5f);
+    // the mean (1/2) folded into the gains
+    const float sY2 = 0.5f * sY, bX2 = 0.5f * P.betaX;
+    a = wiener(a, sX, sY2, bX2, 0.5f);
+    bb = wiener(bb, sX, sY2, bX2, 0.5f);
+    f = cadd(a, bb);
     if (dg) f = cadd(f, d);
   } else if constexpr (BM == 3) {
     const c32 pv = v[0], cu = v[1], nx = v[2];
@@ -406,11 +413,13 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
     c32 fp1 = cmake(d02.x - d13.y, d02.y + d13.x);  // (c - p2) + i (n - p1)
     c32 fn = cmake(d02.x + d13.y, d02.y - d13.x);   // (c - p2) - i (n - p1)
     if (dg) fp1 = csub(fp1, cscale(d, 4.0f));
-    fp2 = wiener(fp2, sX, sY, P.betaX);
-    fp1 = wiener(fp1, sX, sY, P.betaX);
-    fc = wiener(fc, sX, sY, P.betaX);
-    fn = wiener(fn, sX, sY, P.betaX);
-    f = cscale(cadd(cadd(cadd(fp2, fp1), fc), fn), 0.25f);
+    // the mean (1/4) folded into the gains
+    const float sY4 = 0.25f * sY, bX4 = 0.25f * P.betaX;
+    fp2 = wiener(fp2, sX, sY4, bX4, 0.25f);
+    fp1 = wiener(fp1, sX, sY4, bX4, 0.25f);
+    fc = wiener(fc, sX, sY4, bX4, 0.25f);
+    fn = wiener(fn, sX, sY4, bX4, 0.25f);
+    f = cadd(cadd(cadd(fp2, fp1), fc), fn);
     if (dg) f = cadd(f, d);
   } else if constexpr (BM == 1) {
     c32 x = v[0];
@@ -441,7 +450,7 @@ __device__ __forceinline__ c32 filter_bin(const PlaneArgs& P, const c32 (&v)[4],
   }
   if (shp) {
     c32 x = dg ? csub(f, d) : f;
-    const float psd = x.x * x.x + x.y * x.y + 1e-37f;
+    const float psd = fmaf(x.x, x.x, fmaf(x.y, x.y, 1e-37f));
     const float q = psd * P.ssmax * frcp((psd + P.ssmin) * (psd + P.ssmax));
     const float gsh = 1.0f + lut * fsqrt(q);
     x = cscale(x, gsh);
