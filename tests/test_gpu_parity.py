@@ -483,12 +483,14 @@ def test_step_host_buffers_equals_host_step(gpu):
         assert np.array_equal(got[c], want[c])
 
 
+@pytest.mark.parametrize("W,H", [(640, 360), (3840, 2160)])
 @pytest.mark.parametrize("dft", [1, 2])
-def test_mfma_spectrum_vs_oracle(gpu, oracle, dft):
+def test_mfma_spectrum_vs_oracle(gpu, oracle, dft, W, H):
     """The MFMA analysis (FP16 or BF16 operands split hi + lo, FP32
     accumulation) against the oracle's FP32 spectra: relative error below
-    2e-6 (FP16) and 2e-5 (BF16) of the largest bin."""
-    W, H = 640, 360
+    2e-6 (FP16) and 2e-5 (BF16) of the largest bin. At 4K the plane has
+    1,029 groups of 32 blocks, more than one round of resident workgroups:
+    the grid is that round, each workgroup with a longer run of blocks."""
     planes = make_clip(41, W, H, 1, "u10")
     from oracle_lib import Plane as OPlane
     kw = dict(bt=3, sigma=3.0)
