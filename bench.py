@@ -314,11 +314,10 @@ def forms_section(dev, frames, steps=100, warmup=10, kernel_steps=30):
 def prefetch_variant(dev, steps=100, warmup=10):
     """The same device-resident workload with each request prefetching the
     next one's new frame (fft3d_plane_prefetch with a device source): that
-    frame's analysis runs on the plane's upload stream beside the current
-    sweep. Frames/s of both forms, and the largest difference from plain
-    requests over 8 frames, in LSB (0: the same kernels on the same data).
-    At 4K it is slower: the concurrent analysis takes the CU slots that the
-    sweep's two workgroups per CU need."""
+    frame's analysis runs on the plane's upload stream after the current
+    sweep, beside its seam combine. Frames/s of both forms, and the largest
+    difference from plain requests over 8 frames, in LSB (0: the same
+    kernels on the same data)."""
     import numpy as np
     import torch
     from fft3dcl_amd.filter import GRAY, Instance

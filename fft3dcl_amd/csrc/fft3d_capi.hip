@@ -244,6 +244,11 @@ struct fft3d_plane {
   bool pendOnDevice = false;
   hipEvent_t callStart = nullptr;     // start of the call that runs the prefetch
   bool callStarted = false;
+  // after that call's filter kernel (the sweep): the prefetch's analysis
+  // starts there, beside the seam combine or synthesis and the readback,
+  // instead of taking CU slots from the sweep
+  hipEvent_t filterDone = nullptr;
+  bool filterMarked = false;
   float* dBlocks = nullptr;  // windowed blocks awaiting overlap-add (unfused path)
   float4* dKalman = nullptr; // bm == 0 state (FP16 pairs with the FP16 ring)
   size_t kalmanBytes = 0;
@@ -322,6 +327,7 @@ struct fft3d_plane {
     for (auto e : evSpare) (void)hipEventDestroy(e);
     if (lastCall) (void)hipEventDestroy(lastCall);
     if (callStart) (void)hipEventDestroy(callStart);
+    if (filterDone) (void)hipEventDestroy(filterDone);
     if (upDone) (void)hipEventDestroy(upDone);
     if (up) (void)hipStreamDestroy(up);
     (void)hipFree(dBlocks);
@@ -631,7 +637,8 @@ struct fft3d_plane {
     bool ok = hipDeviceSynchronize() == hipSuccess &&
               hipStreamCreateWithFlags(&up, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&upDone, hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&callStart, hipEventDisableTiming) == hipSuccess;
+              hipEventCreateWithFlags(&callStart, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&filterDone, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventRecord(upDone, up) == hipSuccess;
     if (!ok) {
       (void)hipGetLastError();
@@ -668,6 +675,14 @@ struct fft3d_plane {
     prof_end(t0, K_ANALYSIS, s);
   }
 
+  // with a prefetch pending, marks the end of the call's filter kernel on s
+  void mark_filter(hipStream_t s) {
+    filterMarked = false;
+    if (pendFn < 0 || !up) return;
+    if (hipEventRecord(filterDone, s) == hipSuccess) filterMarked = true;
+    else (void)hipGetLastError();
+  }
+
   bool launched(const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -702,6 +717,7 @@ const char* fft3d_plane::launch(const fk::WindowArgs& w, void* outp, hipStream_t
               a);
     prof_end(t0, fwdMask ? K_FSWEEP : K_SWEEP, s);
     if (!ok()) return fwdMask ? "analysis/filter/synthesis sweep" : "filter/synthesis sweep";
+    mark_filter(s);
     if (p.bm == 0) kalmanFirst = false;
     if (a.nseams) {
       CombineFn kc = &fk::k_combine_seams;
@@ -717,6 +733,7 @@ const char* fft3d_plane::launch(const fk::WindowArgs& w, void* outp, hipStream_t
     FK_LAUNCH(kf, dim3(grid), dim3(ks.fftThreads), s, args, w, dKalman, kfirst, blocks);
     prof_end(t0, K_FILTER, s);
     if (!ok()) return "filter/inverse";
+    mark_filter(s);
     if (p.bm == 0) kalmanFirst = false;
     SynthFn ksy = &fk::k_synthesize;
     t0 = begin();
@@ -750,18 +767,19 @@ int fft3d_plane::run(const fk::WindowArgs& w, void* dst, int dst_on_device, hipS
 
 // The pending prefetch: frame pendFn from plane pendSrc into the ring on the
 // upload stream. Runs inside a process call, after its launches, so that the
-// host copies overlap that call's kernels and the upload (or, for a device
-// source, the analysis) overlaps its kernels and readback. A failure only
-// drops the request (the frame is then forwarded when a window first needs
-// it).
+// host copies and the upload overlap that call's kernels, and the analysis
+// its seam combine (or synthesis) and readback: the analysis waits for the
+// call's sweep, whose CU slots it would otherwise take. A failure only drops
+// the request (the frame is then forwarded when a window first needs it).
 void fft3d_plane::run_prefetch() {
   const int fn = pendFn;
   const void* src = pendSrc;
   const bool onDevice = pendOnDevice;
-  const bool started = callStarted;
+  const bool started = callStarted, marked = filterMarked;
   pendFn = -1;
   pendSrc = nullptr;
   callStarted = false;
+  filterMarked = false;
   if (fn < 0 || ring.count(fn) || !ensure_up()) return;
   const size_t need = (size_t)d.width * d.height * bps;
   // the plane's earlier calls may still read a pooled spectrum; a device
@@ -798,6 +816,8 @@ void fft3d_plane::run_prefetch() {
                : (staged ? stgUp.upload(dStageUp, src, need, up)
                          : hipMemcpyAsync(dStageUp, src, need, hipMemcpyHostToDevice, up));
   if (e != hipSuccess) return drop();
+  // the upload overlaps the call's kernels; the analysis waits for its sweep
+  if (marked && hipStreamWaitEvent(up, filterDone, 0) != hipSuccess) return drop();
   const void* plane = onDevice ? src : dStageUp;
   launch_analysis(plane, spec, up);
   if (hipGetLastError() != hipSuccess) return drop();
