@@ -956,15 +956,10 @@ struct StepEngine {
     }
     return true;
   }
-  // streams of the kernel classes of a step call, fork and join events
+  // streams of the kernel classes of a step call and their join events
   std::vector<hipStream_t> cls;
   std::vector<hipEvent_t> join;
-  hipEvent_t fork = nullptr;
   bool class_streams(size_t n) {
-    if (!fork && hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) {
-      (void)hipGetLastError();
-      return false;
-    }
     while (cls.size() < n) {
       hipStream_t st = nullptr;
       hipEvent_t ev = nullptr;
@@ -995,22 +990,31 @@ struct StepEngine {
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
     graphs.clear();
   }
-  // Device copy of the n descriptors (the previous step's kernels on bs are
-  // done with the buffer by the time the copy runs: same stream).
-  bool upload(const std::vector<fk::BatchItem>& v) {
-    const size_t n = v.size();
-    if (n > capItems) {
-      if (hipStreamSynchronize(bs) != hipSuccess) return false;
-      drop_graphs();  // their kernel arguments point at the old buffer
-      (void)hipFree(dItems);
-      dItems = nullptr;
-      capItems = std::max<size_t>(256, 2 * n);
-      if (hipMalloc(reinterpret_cast<void**>(&dItems), capItems * sizeof(fk::BatchItem)) !=
-          hipSuccess) {
-        capItems = 0;
-        return false;
-      }
+  // A descriptor buffer of at least n entries for a step call (before its
+  // first upload; a new buffer drops the graphs, whose kernel arguments
+  // point at the old one).
+  bool reserve(size_t n) {
+    if (n <= capItems) return true;
+    if (hipStreamSynchronize(bs) != hipSuccess) return false;
+    for (hipStream_t c : cls)
+      if (hipStreamSynchronize(c) != hipSuccess) return false;
+    drop_graphs();
+    (void)hipFree(dItems);
+    dItems = nullptr;
+    capItems = std::max<size_t>(256, 2 * n);
+    if (hipMalloc(reinterpret_cast<void**>(&dItems), capItems * sizeof(fk::BatchItem)) !=
+        hipSuccess) {
+      capItems = 0;
+      return false;
     }
+    return true;
+  }
+  // Device copy of descriptors into [at, at + n) on bs, through a pinned
+  // stage (the previous step's kernels are done with the buffer by the time
+  // the copy runs: bs waits for them).
+  bool upload(const std::vector<fk::BatchItem>& v, size_t at) {
+    const size_t n = v.size();
+    if (at + n > capItems) return false;
     const int k = nextStage;
     nextStage = (nextStage + 1) % NSTAGE;
     if (hUsed[k] && hipEventSynchronize(hEv[k]) != hipSuccess) return false;
@@ -1025,11 +1029,25 @@ struct StepEngine {
       }
     }
     std::memcpy(hItems[k], v.data(), n * sizeof(fk::BatchItem));
-    if (hipMemcpyAsync(dItems, hItems[k], n * sizeof(fk::BatchItem), hipMemcpyHostToDevice,
-                       bs) != hipSuccess ||
+    if (hipMemcpyAsync(dItems + at, hItems[k], n * sizeof(fk::BatchItem),
+                       hipMemcpyHostToDevice, bs) != hipSuccess ||
         hipEventRecord(hEv[k], bs) != hipSuccess)
       return false;
     hUsed[k] = true;
+    return true;
+  }
+  // the event that marks chunk i's descriptors in place (created on first use)
+  std::vector<hipEvent_t> chunkEvs;
+  bool chunk_event(size_t i, hipEvent_t& e) {
+    while (chunkEvs.size() <= i) {
+      hipEvent_t x = nullptr;
+      if (hipEventCreateWithFlags(&x, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+      }
+      chunkEvs.push_back(x);
+    }
+    e = chunkEvs[i];
     return true;
   }
 };
@@ -1060,12 +1078,14 @@ void batch_add(ClassRound& cr, int which, fk::BatchFn fn, int threads, fk::Batch
 
 // The batchable items of a step call, round by round (round r: each plane's
 // r-th item). Each kernel class runs its rounds in order on a stream of its
-// own, forked from the step stream and joined back, so classes overlap; the
-// step stream starts after every item's stream and each plane's previous
-// call, and every item's stream continues after the step. With graphs on,
-// the whole launch sequence is one hipGraph (a branch per class), built once
-// per sequence shape and replayed; the descriptors are uploaded before each
-// launch in one copy.
+// own, started after the step stream and joined back, so classes overlap;
+// the step stream starts after every item's stream and each plane's previous
+// call, and every item's stream continues after the step. The rounds go to
+// the GPU in chunks of 1, 2, 4, ... rounds, so that the host plans a chunk
+// while the GPU runs the ones before it: each chunk's descriptors are copied
+// into a range of their own, and each class's share of the chunk follows
+// them on the class stream. With graphs on, that share is a hipGraph, built
+// once per shape and replayed.
 void step_batch(fft3d_work_item* items, const std::vector<std::vector<int>>& rounds) {
   StepEngine& E = g_step;
   std::lock_guard<std::mutex> lk(E.mu);
@@ -1109,9 +1129,93 @@ void step_batch(fft3d_work_item* items, const std::vector<std::vector<int>>& rou
     }
   } release{items, all_items};
   std::vector<std::pair<fk::BatchFn, int>> classes;  // sweep kernel, plane size range
-  std::vector<std::vector<ClassRound>> plan;   // [class][round]
-  std::vector<std::pair<fft3d_plane*, int>> added;  // ring entries the analyses fill
-  for (size_t r = 0; r < rounds.size(); r++) {
+  std::vector<std::vector<ClassRound>> plan;   // [class][round of the chunk]
+  struct Added {  // a ring entry an analysis fills, and the chunk that launches it
+    fft3d_plane* p;
+    int fn, chunk;
+  };
+  std::vector<Added> added;
+  std::vector<char> clsUsed;  // class streams that received work
+  // descriptors for every chunk of the call, each chunk in its own range
+  size_t maxDesc = 0;
+  for (int k : all_items) maxDesc += 6;  // <= 4 analyses, a sweep, a combine
+  bool good = E.reserve(maxDesc);
+  size_t base = 0;
+  int chunk = 0, failedChunk = -1;
+  // Rounds go to the GPU in chunks of 1, 2, 4, ... rounds: the host plans a
+  // chunk while the GPU runs the ones before it. Each class's share of a
+  // chunk goes onto the class's stream after the chunk's descriptors.
+  auto flush = [&]() -> bool {
+    std::vector<fk::BatchItem> desc;
+    std::vector<std::vector<BatchLaunch>> seq(plan.size());
+    for (size_t c = 0; c < plan.size(); c++)
+      for (ClassRound& cr : plan[c])
+        for (int q = 0; q < 3; q++) {
+          if (cr.d[q].empty()) continue;
+          cr.l[q].first = base + desc.size();
+          cr.l[q].n = cr.d[q].size();
+          desc.insert(desc.end(), cr.d[q].begin(), cr.d[q].end());
+          seq[c].push_back(cr.l[q]);
+        }
+    for (auto& pc : plan) pc.clear();
+    if (desc.empty()) return true;
+    hipEvent_t ready = nullptr;
+    if (!E.class_streams(seq.size()) || !E.chunk_event(chunk, ready) ||
+        !E.upload(desc, base) || hipEventRecord(ready, bs) != hipSuccess)
+      return false;
+    base += desc.size();
+    if (clsUsed.size() < seq.size()) clsUsed.resize(seq.size(), 0);
+    for (size_t c = 0; c < seq.size(); c++) {
+      if (seq[c].empty()) continue;
+      hipStream_t cs = E.cls[c];
+      if (hipStreamWaitEvent(cs, ready, 0) != hipSuccess) return false;
+      clsUsed[c] = 1;
+      auto launch_class = [&]() {
+        for (const BatchLaunch& l : seq[c])
+          FK_LAUNCH(l.fn, dim3(l.grid), dim3(l.threads), cs, E.dItems + l.first, (int)l.n);
+        return hipGetLastError() == hipSuccess;
+      };
+      if (!E.graphsOn) {
+        if (!launch_class()) return false;
+        continue;
+      }
+      // a capture must not be the first use of a kernel (its code object is
+      // loaded then): query each kernel once beforehand
+      std::vector<uintptr_t> key{reinterpret_cast<uintptr_t>(E.dItems), ~(uintptr_t)c};
+      for (const BatchLaunch& l : seq[c]) {
+        const void* f = reinterpret_cast<const void*>(l.fn);
+        if (std::find(E.loaded.begin(), E.loaded.end(), f) == E.loaded.end()) {
+          hipFuncAttributes attr;
+          if (hipFuncGetAttributes(&attr, f) != hipSuccess) (void)hipGetLastError();
+          else E.loaded.push_back(f);
+        }
+        key.insert(key.end(), {reinterpret_cast<uintptr_t>(l.fn), (uintptr_t)l.grid,
+                               (uintptr_t)l.threads, (uintptr_t)l.first, (uintptr_t)l.n});
+      }
+      auto g = E.graphs.find(key);
+      if (g == E.graphs.end()) {
+        if (E.graphs.size() >= 256) E.drop_graphs();
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        bool built = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (built) {
+          built = launch_class();
+          built = hipStreamEndCapture(cs, &graph) == hipSuccess && built;
+        }
+        built = built && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!built) return false;
+        g = E.graphs.emplace(key, exec).first;
+        E.graphBuilds++;
+      } else {
+        E.graphHits++;
+      }
+      if (hipGraphLaunch(g->second, cs) != hipSuccess) return false;
+    }
+    return true;
+  };
+  size_t r0 = 0, span = 1;
+  for (size_t r = 0; r < rounds.size() && good; r++) {
     for (int k : rounds[r]) {
       fft3d_work_item& it = items[k];
       fft3d_plane* p = it.plane;
@@ -1168,10 +1272,10 @@ void step_batch(fft3d_work_item* items, const std::vector<std::vector<int>>& rou
         classes.push_back(key);
         plan.emplace_back();
       }
-      if (plan[c].size() <= r) plan[c].resize(r + 1);
-      ClassRound& cr = plan[c][r];
+      if (plan[c].size() <= r - r0) plan[c].resize(r - r0 + 1);
+      ClassRound& cr = plan[c][r - r0];
       for (int j = 0; j < nmiss; j++) {
-        added.emplace_back(p, missing[j]);
+        added.push_back({p, missing[j], chunk});
         fk::BatchItem bi{};
         bi.P = p->args;
         bi.plane = src[j];
@@ -1200,90 +1304,39 @@ void step_batch(fft3d_work_item* items, const std::vector<std::vector<int>>& rou
       if (p->p.bm == 0) p->kalmanFirst = false;
       step_prune(p, it.frame);
     }
-  }
-  // one descriptor array for the whole call, in launch order
-  std::vector<fk::BatchItem> desc;
-  std::vector<std::vector<BatchLaunch>> seq(plan.size());
-  for (size_t c = 0; c < plan.size(); c++)
-    for (ClassRound& cr : plan[c])
-      for (int q = 0; q < 3; q++) {
-        if (cr.d[q].empty()) continue;
-        cr.l[q].first = desc.size();
-        cr.l[q].n = cr.d[q].size();
-        desc.insert(desc.end(), cr.d[q].begin(), cr.d[q].end());
-        seq[c].push_back(cr.l[q]);
+    if (r + 1 == r0 + span || r + 1 == rounds.size()) {
+      if (!flush()) {
+        (void)hipGetLastError();
+        failedChunk = chunk;
+        good = false;
       }
-  bool ok = desc.empty() || E.upload(desc);
-  if (ok && !desc.empty()) {
-    ok = E.class_streams(plan.size());
-    // fork to the class streams, their launches, join back to the step stream
-    auto launch_all = [&]() {
-      if (hipEventRecord(E.fork, bs) != hipSuccess) return false;
-      for (size_t c = 0; c < seq.size(); c++) {
-        hipStream_t cs = E.cls[c];
-        if (hipStreamWaitEvent(cs, E.fork, 0) != hipSuccess) return false;
-        for (const BatchLaunch& l : seq[c])
-          FK_LAUNCH(l.fn, dim3(l.grid), dim3(l.threads), cs, E.dItems + l.first, (int)l.n);
-        if (hipEventRecord(E.join[c], cs) != hipSuccess ||
-            hipStreamWaitEvent(bs, E.join[c], 0) != hipSuccess)
-          return false;
-      }
-      return hipGetLastError() == hipSuccess;
-    };
-    if (!ok) {
-    } else if (!E.graphsOn) {
-      ok = launch_all();
-    } else {
-      // a capture must not be the first use of a kernel (its code object
-      // is loaded then): query each kernel once beforehand
-      std::vector<uintptr_t> key{reinterpret_cast<uintptr_t>(E.dItems)};
-      for (size_t c = 0; c < seq.size(); c++) {
-        key.push_back(~(uintptr_t)c);
-        for (const BatchLaunch& l : seq[c]) {
-          const void* f = reinterpret_cast<const void*>(l.fn);
-          if (std::find(E.loaded.begin(), E.loaded.end(), f) == E.loaded.end()) {
-            hipFuncAttributes attr;
-            if (hipFuncGetAttributes(&attr, f) != hipSuccess) (void)hipGetLastError();
-            else E.loaded.push_back(f);
-          }
-          key.insert(key.end(), {reinterpret_cast<uintptr_t>(l.fn), (uintptr_t)l.grid,
-                                 (uintptr_t)l.threads, (uintptr_t)l.first, (uintptr_t)l.n});
-        }
-      }
-      auto g = E.graphs.find(key);
-      if (g == E.graphs.end()) {
-        if (E.graphs.size() >= 256) E.drop_graphs();
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        ok = hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal) == hipSuccess;
-        if (ok) {
-          ok = launch_all();
-          ok = hipStreamEndCapture(bs, &graph) == hipSuccess && ok;
-        }
-        ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-        if (graph) (void)hipGraphDestroy(graph);
-        if (ok) {
-          g = E.graphs.emplace(key, exec).first;
-          E.graphBuilds++;
-        }
-      } else {
-        E.graphHits++;
-      }
-      ok = ok && hipGraphLaunch(g->second, bs) == hipSuccess;
+      chunk++;
+      r0 = r + 1;
+      span *= 2;
     }
   }
-  if (!ok) {
+  if (!good) {
     (void)hipGetLastError();
-    // the analyses did not run: their ring entries hold nothing
-    for (auto& pa : added) {
-      auto r = pa.first->ring.find(pa.second);
-      if (r != pa.first->ring.end()) {
-        pa.first->retire(r->second);
-        pa.first->ring.erase(r);
+    // the analyses of the failed chunk and after did not run: their ring
+    // entries hold nothing
+    for (const Added& a : added) {
+      if (failedChunk >= 0 && a.chunk < failedChunk) continue;
+      auto r = a.p->ring.find(a.fn);
+      if (r != a.p->ring.end()) {
+        a.p->retire(r->second);
+        a.p->ring.erase(r);
       }
     }
-    return fail_all(FFT3D_ELAUNCH, "batched step launch failed");
   }
+  // the step stream continues after every class stream's work
+  for (size_t c = 0; c < clsUsed.size(); c++)
+    if (clsUsed[c] && (hipEventRecord(E.join[c], E.cls[c]) != hipSuccess ||
+                       hipStreamWaitEvent(bs, E.join[c], 0) != hipSuccess)) {
+      (void)hipGetLastError();
+      good = false;
+    }
+  if (!good) fail_all(FFT3D_ELAUNCH, "batched step launch failed");
+
   // later work on each item's stream comes after the step
   if (hipEventRecord(E.done, bs) != hipSuccess) {
     (void)hipGetLastError();

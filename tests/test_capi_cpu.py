@@ -716,12 +716,14 @@ def _raw_step(lib, reqs):
     return [items[k].status for k in range(len(reqs))]
 
 
-def test_emu_batched_step_equals_item_step(emu_lib):
+@pytest.mark.parametrize("groups", [((0, 1), (2, 3), (4,)), ((0, 1, 2, 3, 4),)])
+def test_emu_batched_step_equals_item_step(emu_lib, groups):
     """The batched step (device buffers: one launch per kernel class over all
     items of a round, each plane's items in rounds) gives the frames of the
     item-by-item step (host buffers), bit for bit, over planes of several
     block sizes, temporal modes, ring precisions and DFT forms, with two
-    frames of one plane in the same call."""
+    frames of one plane in the same call, or all five (rounds launched in
+    chunks of 1, 2 and 2)."""
     n = 5
     cfgs = [(96, 64, 8, dict(bt=4, sharpen=0.4), {}),
             (80, 72, 10, dict(bt=3, bw=16, bh=16), {}),
@@ -741,8 +743,7 @@ def test_emu_batched_step_equals_item_step(emu_lib):
             planes.append(pl)
             clips.append(make_clip(60 + c, W, H, n, "u8" if bits == 8 else "u10")[0])
             outs.append(np.zeros_like(clips[-1]))
-        for f0 in (0, 2, 4):
-            frames = [f for f in (f0, f0 + 1) if f < n]
+        for frames in groups:
             reqs = [(planes[c], f, n, clips[c], outs[c][f], dev) for f in frames
                     for c in range(len(cfgs))]
             assert _raw_step(emu_lib, reqs) == [0] * len(reqs)
