@@ -131,6 +131,9 @@ struct Shape {
 // Direct3D MIRROR addressing (src/cpu/reference.cpp:10-17).
 __device__ __forceinline__ int mirror(int x, int w) {
   if (x >= 0 && x < w) return x;
+  // one reflection covers every block of a plane at least a block wide: no
+  // integer division on the edge blocks
+  if (x >= -w && x < 2 * w) return x < 0 ? -x - 1 : 2 * w - 1 - x;
   int p = x % (2 * w);
   if (p < 0) p += 2 * w;
   if (p >= w) p = 2 * w - 1 - p;
