@@ -86,21 +86,44 @@ class Instance:
                 o.set_option(capi.Plane.OPT_DFT, dft)
         info = self.objs[0].info
         self.bm, self.bt, self.current = info.bm, info.bt, info.current
+        self._conc = None  # per-plane streams and events of device requests
 
     def window(self, n: int) -> list[int]:
         if self.bm >= 2:
             return [clamp_frame(n - self.current + i, self.num_frames) for i in range(self.bt)]
         return [clamp_frame(n, self.num_frames)]
 
+    def _plane_streams(self, stream):
+        """With device buffers and several processed planes, each plane's
+        calls go on a stream of its own, after the caller's stream (where the
+        sources were written) and before the caller's later work, so the
+        planes of a frame overlap on the GPU; the reference runs them one
+        after another (src/fft3dcl.cpp:431-448)."""
+        import torch
+        if self._conc is None:
+            self._conc = dict(fork=torch.cuda.Event(),
+                              planes=[(torch.cuda.Stream(), torch.cuda.Event())
+                                      for _ in self.planes])
+        caller = (torch.cuda.ExternalStream(stream) if stream
+                  else torch.cuda.current_stream())
+        self._conc["fork"].record(caller)
+        for ps, _ in self._conc["planes"]:
+            ps.wait_event(self._conc["fork"])
+        return caller
+
     def get_frame(self, n: int, src_planes: Sequence, dst_planes: list,
                   on_device: bool = False, stream=None) -> None:
         """src_planes[q](fn) -> the source plane q of frame fn (numpy or device
-        tensor); dst_planes[q] receives plane q of output frame n."""
+        tensor); dst_planes[q] receives plane q of output frame n. With device
+        buffers the processed planes run concurrently, each on a stream of its
+        own between the caller's earlier and later work on `stream`."""
         for q in range(self.nplanes):
             if q not in self.planes:
                 dst_planes[q][...] = src_planes[q](n)
+        caller = self._plane_streams(stream) if on_device and len(self.planes) > 1 else None
         for k, q in enumerate(self.planes):
             obj = self.objs[k]
+            s = self._conc["planes"][k][0].cuda_stream if caller is not None else stream
             win = self.window(n)
             if self.prefetch and self.bm >= 2:
                 # the next sequential request's new frame (kept by the prune below)
@@ -109,19 +132,23 @@ class Instance:
                     obj.prefetch(nxt, src_planes[q](nxt), on_device)
             missing = [fn for fn in dict.fromkeys(win) if not obj.has(fn)]
             for fn in missing[:-1]:
-                obj.forward(fn, src_planes[q](fn), on_device, stream)
+                obj.forward(fn, src_planes[q](fn), on_device, s)
             if missing:
                 fn = missing[-1]
                 obj.forward_process(fn, src_planes[q](fn), win, dst_planes[q], on_device,
-                                    on_device, stream)
+                                    on_device, s)
             else:
-                obj.process(win, dst_planes[q], on_device, stream)
+                obj.process(win, dst_planes[q], on_device, s)
             if self.bm >= 2:
                 obj.prune(n - self.current - 1, n - self.current + self.bt)
             else:
                 # one-frame modes and Kalman (whose state is not in the ring)
                 # reuse no spectrum; the reference keeps bt=0 spectra forever
                 obj.clear()
+            if caller is not None:
+                ps, done = self._conc["planes"][k]
+                done.record(ps)
+                caller.wait_event(done)
 
     def close(self):
         for o in self.objs:

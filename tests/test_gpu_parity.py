@@ -198,6 +198,29 @@ def test_device_resident_path_equals_host_path(gpu):
     assert np.array_equal(dev_out.cpu().numpy().view(np.uint16), host)
 
 
+@pytest.mark.parametrize("prefetch", [False, True])
+def test_device_yuv_planes_on_their_streams_equal_host_path(gpu, prefetch):
+    """With device buffers the three planes of a 4:2:0 frame run on streams
+    of their own, between the caller's earlier and later work; the frames
+    equal the host-buffer path's (planes one after another) bit for bit."""
+    W, H, n = 640, 360, 6
+    planes = make_clip(77, W, H, n, "u10", color="yuv")
+    kw = dict(bt=4, sharpen=0.4, plane=4)
+    host = FFT3DCL(planes, color_family=3, bits=10, lib=gpu, **kw)
+    dev_in = [torch.from_numpy(p).cuda() for p in planes]
+    dev_out = [torch.zeros(p.shape, dtype=torch.int16, device="cuda") for p in planes]
+    stream = torch.cuda.Stream()
+    inst = Instance(3, W, H, 10, 0, n, lib=gpu, prefetch=prefetch, **kw)
+    srcs = [lambda fn, d=d: d[fn] for d in dev_in]
+    for k in range(n):
+        inst.get_frame(k, srcs, [o[k] for o in dev_out], on_device=True,
+                       stream=stream.cuda_stream)
+    stream.synchronize()
+    inst.close()
+    for q in range(3):
+        assert np.array_equal(dev_out[q].cpu().numpy().view(np.uint16), host[q]), q
+
+
 @pytest.mark.parametrize("where,bt,half", [("pageable", 4, False), ("pinned", 4, False),
                                            ("device", 4, False), ("pageable", 3, False),
                                            ("pinned", 2, True)])
