@@ -4,7 +4,8 @@ out-of-bounds global or LDS access in the kernels' index arithmetic fails on
 the CPU before it could fault a GPU. The host buffers go through the pinned
 chunks of the staged transfers (ragged chunks, and the threaded copy of a
 plane above 256 KiB), and with the sweep on each request prefetches the next
-frame. LDS arrays are function statics in the
+frame; a batched step call of five frames of two planes covers the step's
+host side. LDS arrays are function statics in the
 emulation, so their overruns are caught as global-buffer overflows."""
 import os
 import subprocess
@@ -45,6 +46,22 @@ out = np.zeros_like(planes[0])
 for n in range(2):
     inst.get_frame(n, [lambda fn: np.ascontiguousarray(planes[0][fn])], [out[n]])
 inst.close()
+# the batched step's host side (descriptors of rounds launched in chunks):
+# five frames of two planes in one call, device flags on host buffers (the
+# emulation's device memory is host memory)
+from test_capi_cpu import _raw_step
+pls, reqs = [], []
+for c, kw in enumerate((dict(bt=4), dict(bt=3, bw=16, bh=16))):
+    pl = capi.Plane(capi.make_params(**kw), 96, 64, 8, lib=lib)
+    clip = make_clip(90 + c, 96, 64, 5, "u8")[0]
+    out = np.zeros_like(clip)
+    pls.append((pl, clip, out))
+    reqs += [(pl, f, 5, clip, out[f], True) for f in range(5)]
+reqs.sort(key=lambda r: r[1])
+assert _raw_step(lib, reqs) == [0] * len(reqs)
+for pl, _, out in pls:
+    assert out.any()
+    pl.close()
 print("asan-ok")
 """
 
