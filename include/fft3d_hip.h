@@ -159,11 +159,13 @@ int fft3d_plane_process_noalloc(fft3d_plane* p, const int* window, int nwin, voi
  * carries it out after launching its own kernels and before its readback,
  * on the plane's own upload stream:
  *   host src   : the host copy (pageable buffers go through pinned chunks
- *                that hold the whole plane) overlaps that call's kernels;
- *                the upload and the analysis run alongside its readback;
- *   device src : the analysis runs alongside that call's kernels. It waits
- *                for the work the caller queued on that call's stream before
- *                the call, so src may be written there just before.
+ *                that hold the whole plane) and the upload overlap that
+ *                call's kernels;
+ *   device src : the source is read after the work the caller queued on
+ *                that call's stream before the call, so src may be written
+ *                there just before.
+ * In both cases the analysis starts after the call's sweep (or filter
+ * kernel) and runs beside its seam combine (or synthesis) and readback.
  * src must stay valid, and a device src unchanged, until the prefetch is
  * done: until the carrying call returns for a host src, until the next call
  * whose window holds fn for a device src. One request at a time (a new one
