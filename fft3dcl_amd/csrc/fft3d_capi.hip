@@ -658,14 +658,15 @@ struct fft3d_plane {
   }
 
   // Analysis of `plane` into `spec` on stream s (register FFTs, or the MFMA
-  // form: 256-thread workgroups of at least 32 blocks, at most one round of
-  // them on the device).
+  // form: 256-thread workgroups, one 4-block chunk each while they fit one
+  // round on the device, else one round of them with longer runs; 32 blocks
+  // each when the resident count is unknown).
   void launch_analysis(const void* plane, fk::c32* spec, hipStream_t s) {
     hipEvent_t t0 = prof_begin(s);
     if (dft) {
       AnalysisFn k = ks.analysisM[dft - 1][half ? 1 : 0];
       int grid = (nb + 31) / 32;
-      if (mfmaPerCU > 0 && ncu > 0) grid = std::min(grid, mfmaPerCU * ncu);
+      if (mfmaPerCU > 0 && ncu > 0) grid = std::min((nb + 3) / 4, mfmaPerCU * ncu);
       FK_LAUNCH(k, dim3(grid), dim3(256), s, args, plane, spec);
     } else {
       const int grid = (nb + ks.G - 1) / ks.G;
